@@ -1,0 +1,765 @@
+// C-ABI of the J/K engine: context construction, builds, test hooks.
+//
+// Context construction mirrors make_direct_jk_context (asuka/hf/direct_jk.py:1143-1470):
+//   shell pairs with l_A >= l_B ................ asuka/cueri/shell_pairs.py:72-116
+//   primitive-pair tables (eta, P, cK) ......... asuka/cueri/_eri_rys_cpu.pyx:2066-2129
+//   Schwarz bounds on the device ............... asuka/cueri/screening.py:224-314
+//   Q-sorted, screened quartet rows ............ asuka/hf/direct_jk.py:1228-1246
+// but instead of materialising 8 B/quartet task arrays grouped by class, the
+// quartets of each (bra class, ket class) pair are described by rows: a bra
+// pair and the length of the Q-descending prefix of the ket class list it
+// pairs with. Kernels decode quartets from (row, column) on the fly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/asuka_jk.h"
+#include "jk_internal.h"
+#include "rys_eval.h"
+
+hipError_t launch_eri_fast(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s, bool* handled);
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_CHECK(expr)                                                                       \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess) return fail(ASUKA_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+namespace {
+
+int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+
+struct ClassPairHost {
+  int X, Y;  // bra class >= ket class
+  int la, lb, lc, ld;
+  int row_start, nrows, col_start;
+  long long nwork;
+  long long nprim_quartets;
+  long long nprim_roots;
+};
+
+__global__ void k_fill(double* x, long long n, double v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long* x, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = 0ull;
+}
+
+__global__ void k_copy(double* dst, const double* src, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// X <- (X + X^T)/2 in place; thread (i, j) with j > i handles both entries.
+__global__ void k_symmetrize(double* X, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= n || j >= n || j <= i) return;
+  const double a = X[(long long)i * n + j], c = X[(long long)j * n + i];
+  const double m = 0.5 * (a + c);
+  X[(long long)i * n + j] = m;
+  X[(long long)j * n + i] = m;
+}
+
+// max |D| over each shell-pair block, both orientations (asuka/hf/direct_jk.py:1484-1527).
+__global__ void k_dsp(const double* D, const int* sp_A, const int* sp_B, const int* shell_ao, const int* shell_l, int nsp,
+                      int nao, double* Dsp) {
+  const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sp >= nsp) return;
+  const int A = sp_A[sp], B = sp_B[sp];
+  const int a0 = shell_ao[A], b0 = shell_ao[B];
+  const int nA = ncart_c(shell_l[A]), nB = ncart_c(shell_l[B]);
+  double m = 0.0;
+  for (int i = 0; i < nA; ++i)
+    for (int j = 0; j < nB; ++j) {
+      m = fmax(m, fabs(D[(long long)(a0 + i) * nao + b0 + j]));
+      if (A != B) m = fmax(m, fabs(D[(long long)(b0 + j) * nao + a0 + i]));
+    }
+  Dsp[sp] = m;
+}
+
+__global__ void k_rys_points(int n, long long npts, const double* T, double* x, double* w) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  for (int u = 0; u < n; ++u) rys_root_dev(n, u, T[i], x[i * n + u], w[i * n + u]);
+}
+
+template <typename Tp>
+int dev_upload(Tp** dst, const std::vector<Tp>& src, std::vector<void*>& owned, hipStream_t s) {
+  *dst = nullptr;
+  size_t bytes = sizeof(Tp) * (src.empty() ? 1 : src.size());
+  hipError_t e = hipMalloc((void**)dst, bytes);
+  if (e != hipSuccess) return fail(ASUKA_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  owned.push_back(*dst);
+  if (!src.empty()) {
+    // blocking copy: callers pass temporaries (context construction is one-time)
+    (void)s;
+    e = hipMemcpy(*dst, src.data(), sizeof(Tp) * src.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(ASUKA_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+}  // namespace
+
+hipError_t launch_int1e(const asuka_jk_ctx* ctx, double* S, double* T, double* V, hipStream_t s);
+
+struct asuka_jk_ctx {
+  int device = 0;
+  int nshell = 0, nao = 0, nsp = 0, natm = 0, max_l = 0;
+  double eps_schwarz = 0, eps_density = 0, tau_mixed = 0;
+  std::vector<int> shell_l, shell_ao, shell_atom, sp_A, sp_B, pp_start, pp_count;
+  std::vector<double> shell_xyz, Q, atom_xyz, atom_Z;
+  std::vector<PrimPair> pp;
+  std::vector<ClassPairHost> cps;
+  long long ntasks = 0;
+  DevBasis dev{};
+  int* d_row_pair = nullptr;
+  long long* d_row_off = nullptr;
+  int* d_col_pair = nullptr;
+  double* d_Q = nullptr;
+  double* d_Dsp = nullptr;
+  double* d_atom_xyz = nullptr;
+  double* d_atom_Z = nullptr;
+  int* d_shell_atom = nullptr;
+  int* d_prim_start = nullptr;
+  int* d_prim_num = nullptr;
+  double* d_prim_exp = nullptr;
+  double* d_prim_coef = nullptr;
+  unsigned long long* d_counters = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  long long last_launches = 0;
+  std::vector<void*> owned;
+  ~asuka_jk_ctx() {
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
+    for (void* p : owned) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+int asuka_jk_version(void) { return ASUKA_JK_ABI_VERSION; }
+
+const char* asuka_jk_last_error(void) { return g_err.c_str(); }
+
+int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, void* stream, asuka_jk_ctx** out) {
+  if (!bd || !opts || !out) return fail(ASUKA_E_INVALID, "basis, opts and out must be non-null");
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  const int nshell = bd->nshell;
+  if (nshell <= 0) return fail(ASUKA_E_INVALID, "AO basis has zero shells");
+  if (!bd->shell_cxyz || !bd->shell_prim_start || !bd->shell_nprim || !bd->shell_l || !bd->shell_ao_start ||
+      !bd->prim_exp || !bd->prim_coef)
+    return fail(ASUKA_E_INVALID, "basis arrays must be non-null");
+  if (!(opts->eps_schwarz >= 0.0)) return fail(ASUKA_E_INVALID, "eps_schwarz must be >= 0");
+
+  auto* ctx = new asuka_jk_ctx();
+  std::unique_ptr<asuka_jk_ctx> guard(ctx);
+  HIP_CHECK(hipGetDevice(&ctx->device));
+  ctx->nshell = nshell;
+  ctx->shell_l.assign(bd->shell_l, bd->shell_l + nshell);
+  ctx->shell_ao.assign(bd->shell_ao_start, bd->shell_ao_start + nshell);
+  ctx->shell_xyz.assign(bd->shell_cxyz, bd->shell_cxyz + 3 * nshell);
+  for (int i = 0; i < nshell; ++i) {
+    const int l = ctx->shell_l[i];
+    if (l < 0) return fail(ASUKA_E_INVALID, "shell_l must be >= 0");
+    if (l > ASUKA_LMAX)
+      return fail(ASUKA_E_UNSUPPORTED, "direct J/K supports only l <= " + std::to_string(ASUKA_LMAX));
+    ctx->max_l = std::max(ctx->max_l, l);
+    const int ps = bd->shell_prim_start[i], np = bd->shell_nprim[i];
+    if (np <= 0 || ps < 0 || ps + np > bd->nprim) return fail(ASUKA_E_INVALID, "shell primitive range out of bounds");
+  }
+  ctx->nao = ctx->shell_ao[nshell - 1] + ncart_c(ctx->shell_l[nshell - 1]);
+  ctx->natm = bd->natm;
+  if (bd->natm > 0) {
+    if (!bd->atom_xyz || !bd->atom_charge) return fail(ASUKA_E_INVALID, "atom arrays must be non-null when natm > 0");
+    ctx->atom_xyz.assign(bd->atom_xyz, bd->atom_xyz + 3 * bd->natm);
+    ctx->atom_Z.assign(bd->atom_charge, bd->atom_charge + bd->natm);
+  }
+  if (bd->shell_atom)
+    ctx->shell_atom.assign(bd->shell_atom, bd->shell_atom + nshell);
+  else
+    ctx->shell_atom.assign(nshell, 0);
+  ctx->eps_schwarz = opts->eps_schwarz;
+  ctx->eps_density = opts->eps_density < 0.0 ? std::max(opts->eps_schwarz, 1e-10) : opts->eps_density;
+  ctx->tau_mixed = opts->tau_mixed;
+
+  // ---- shell pairs, l-ordered orientation (asuka/cueri/shell_pairs.py:72-116) ----
+  for (int i = 0; i < nshell; ++i)
+    for (int j = 0; j <= i; ++j) {
+      int A = i, B = j;
+      if (ctx->shell_l[i] < ctx->shell_l[j]) std::swap(A, B);
+      ctx->sp_A.push_back(A);
+      ctx->sp_B.push_back(B);
+    }
+  ctx->nsp = (int)ctx->sp_A.size();
+
+  // ---- primitive pairs (pyx:2066-2129), exact zeros dropped ----
+  for (int sp = 0; sp < ctx->nsp; ++sp) {
+    const int A = ctx->sp_A[sp], B = ctx->sp_B[sp];
+    const double* a3 = &ctx->shell_xyz[3 * A];
+    const double* b3 = &ctx->shell_xyz[3 * B];
+    const double dx = a3[0] - b3[0], dy = a3[1] - b3[1], dz = a3[2] - b3[2];
+    const double AB2 = dx * dx + dy * dy + dz * dz;
+    ctx->pp_start.push_back((int)ctx->pp.size());
+    for (int ia = 0; ia < bd->shell_nprim[A]; ++ia)
+      for (int ib = 0; ib < bd->shell_nprim[B]; ++ib) {
+        const int pA = bd->shell_prim_start[A] + ia, pB = bd->shell_prim_start[B] + ib;
+        const double al = bd->prim_exp[pA], be = bd->prim_exp[pB];
+        const double eta = al + be, ie = 1.0 / eta;
+        PrimPair r;
+        r.eta = eta;
+        r.Px = (al * a3[0] + be * b3[0]) * ie;
+        r.Py = (al * a3[1] + be * b3[1]) * ie;
+        r.Pz = (al * a3[2] + be * b3[2]) * ie;
+        r.cK = bd->prim_coef[pA] * bd->prim_coef[pB] * std::exp(-(al * be) * ie * AB2);
+        r.a_exp = al;
+        r.b_exp = be;
+        r.inv_eta = ie;
+        if (r.cK != 0.0) ctx->pp.push_back(r);
+      }
+    ctx->pp_count.push_back((int)ctx->pp.size() - ctx->pp_start.back());
+  }
+
+  // ---- device copies ----
+  std::vector<void*>& own = ctx->owned;
+  int rc;
+  double* d_xyz;
+  int *d_l, *d_ao, *d_spA, *d_spB, *d_pps, *d_ppc;
+  PrimPair* d_pp;
+  if ((rc = dev_upload(&d_xyz, ctx->shell_xyz, own, s))) return rc;
+  if ((rc = dev_upload(&d_l, ctx->shell_l, own, s))) return rc;
+  if ((rc = dev_upload(&d_ao, ctx->shell_ao, own, s))) return rc;
+  if ((rc = dev_upload(&d_spA, ctx->sp_A, own, s))) return rc;
+  if ((rc = dev_upload(&d_spB, ctx->sp_B, own, s))) return rc;
+  if ((rc = dev_upload(&d_pps, ctx->pp_start, own, s))) return rc;
+  if ((rc = dev_upload(&d_ppc, ctx->pp_count, own, s))) return rc;
+  if ((rc = dev_upload(&d_pp, ctx->pp, own, s))) return rc;
+  if ((rc = dev_upload(&ctx->d_shell_atom, ctx->shell_atom, own, s))) return rc;
+  {
+    std::vector<int> ps(bd->shell_prim_start, bd->shell_prim_start + nshell), pn(bd->shell_nprim, bd->shell_nprim + nshell);
+    std::vector<double> pe(bd->prim_exp, bd->prim_exp + bd->nprim), pc(bd->prim_coef, bd->prim_coef + bd->nprim);
+    if ((rc = dev_upload(&ctx->d_prim_start, ps, own, s))) return rc;
+    if ((rc = dev_upload(&ctx->d_prim_num, pn, own, s))) return rc;
+    if ((rc = dev_upload(&ctx->d_prim_exp, pe, own, s))) return rc;
+    if ((rc = dev_upload(&ctx->d_prim_coef, pc, own, s))) return rc;
+  }
+  std::vector<double> zeros(ctx->nsp, 0.0);
+  if ((rc = dev_upload(&ctx->d_Q, zeros, own, s))) return rc;
+  if ((rc = dev_upload(&ctx->d_Dsp, zeros, own, s))) return rc;
+  if (ctx->natm > 0) {
+    if ((rc = dev_upload(&ctx->d_atom_xyz, ctx->atom_xyz, own, s))) return rc;
+    if ((rc = dev_upload(&ctx->d_atom_Z, ctx->atom_Z, own, s))) return rc;
+  }
+  DevBasis& dv = ctx->dev;
+  dv.shell_xyz = d_xyz;
+  dv.shell_l = d_l;
+  dv.shell_ao = d_ao;
+  dv.sp_A = d_spA;
+  dv.sp_B = d_spB;
+  dv.pp_start = d_pps;
+  dv.pp_count = d_ppc;
+  dv.pp = d_pp;
+  dv.Q = ctx->d_Q;
+  dv.Dsp = ctx->d_Dsp;
+  dv.nao = ctx->nao;
+  dv.nsp = ctx->nsp;
+
+  // ---- Schwarz bounds (device) ----
+  ctx->Q.assign(ctx->nsp, 1.0);
+  if (ctx->eps_schwarz > 0.0) {
+    std::vector<int> diag(ctx->nsp);
+    std::iota(diag.begin(), diag.end(), 0);
+    int* d_diag;
+    if ((rc = dev_upload(&d_diag, diag, own, s))) return rc;
+    ClassWork w{};
+    w.la = w.lb = w.lc = w.ld = ctx->max_l;
+    w.nwork = ctx->nsp;
+    w.q_ab = d_diag;
+    w.q_cd = d_diag;
+    EriOut o{};
+    o.mode = MODE_SCHWARZ;
+    o.Qout = ctx->d_Q;
+    HIP_CHECK(launch_eri_generic(dv, w, o, s));
+    HIP_CHECK(hipMemcpyAsync(ctx->Q.data(), ctx->d_Q, sizeof(double) * ctx->nsp, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    HIP_CHECK(hipMemcpy(ctx->d_Q, ctx->Q.data(), sizeof(double) * ctx->nsp, hipMemcpyHostToDevice));
+  }
+
+  // ---- class lists: pairs of each class, Q descending (stable), Q > 0 ----
+  const int ncls = pair_class(ASUKA_LMAX, ASUKA_LMAX) + 1;
+  std::vector<std::vector<int>> cls_pairs(ncls);
+  for (int sp = 0; sp < ctx->nsp; ++sp) {
+    if (!(ctx->Q[sp] > 0.0)) continue;
+    cls_pairs[pair_class(ctx->shell_l[ctx->sp_A[sp]], ctx->shell_l[ctx->sp_B[sp]])].push_back(sp);
+  }
+  for (auto& v : cls_pairs)
+    std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return ctx->Q[a] > ctx->Q[b]; });
+  // per-class prefix sums of primitive-pair counts (for statistics)
+  std::vector<std::vector<long long>> cls_kpre(ncls);
+  for (int c = 0; c < ncls; ++c) {
+    cls_kpre[c].assign(cls_pairs[c].size() + 1, 0);
+    for (size_t i = 0; i < cls_pairs[c].size(); ++i) cls_kpre[c][i + 1] = cls_kpre[c][i] + ctx->pp_count[cls_pairs[c][i]];
+  }
+  std::vector<int> cls_la(ncls), cls_lb(ncls);
+  for (int la = 0; la <= ASUKA_LMAX; ++la)
+    for (int lb = 0; lb <= la; ++lb) {
+      cls_la[pair_class(la, lb)] = la;
+      cls_lb[pair_class(la, lb)] = lb;
+    }
+  std::vector<int> col_all, row_pair_all;
+  std::vector<long long> row_off_all;
+  std::vector<int> col_start(ncls, 0);
+  for (int c = 0; c < ncls; ++c) {
+    col_start[c] = (int)col_all.size();
+    col_all.insert(col_all.end(), cls_pairs[c].begin(), cls_pairs[c].end());
+  }
+  const double eps = ctx->eps_schwarz;
+  for (int X = 0; X < ncls; ++X) {
+    for (int Y = 0; Y <= X; ++Y) {
+      const auto& rows = cls_pairs[X];
+      const auto& cols = cls_pairs[Y];
+      if (rows.empty() || cols.empty()) continue;
+      ClassPairHost cp{};
+      cp.X = X;
+      cp.Y = Y;
+      cp.la = cls_la[X];
+      cp.lb = cls_lb[X];
+      cp.lc = cls_la[Y];
+      cp.ld = cls_lb[Y];
+      cp.row_start = (int)row_pair_all.size();
+      cp.col_start = col_start[Y];
+      const int nroots = (cp.la + cp.lb + cp.lc + cp.ld) / 2 + 1;
+      long long off = 0;
+      std::vector<long long> offs;
+      for (size_t r = 0; r < rows.size(); ++r) {
+        const int P = rows[r];
+        long long len;
+        if (eps > 0.0) {
+          const double thr = eps / std::max(ctx->Q[P], 1e-300);
+          // number of ket pairs with Q >= thr (cols are Q-descending)
+          len = std::partition_point(cols.begin(), cols.end(), [&](int c) { return ctx->Q[c] >= thr; }) - cols.begin();
+        } else {
+          len = (long long)cols.size();
+        }
+        if (X == Y) len = std::min<long long>(len, (long long)r + 1);
+        if (len <= 0) continue;
+        row_pair_all.push_back(P);
+        offs.push_back(off);
+        off += len;
+        cp.nprim_quartets += (long long)ctx->pp_count[P] * cls_kpre[Y][len];
+      }
+      if (off == 0) continue;
+      cp.nrows = (int)offs.size();
+      cp.nwork = off;
+      cp.nprim_roots = cp.nprim_quartets * nroots;
+      // row offsets are stored per class pair with a trailing total
+      const int base = (int)row_off_all.size();
+      row_off_all.insert(row_off_all.end(), offs.begin(), offs.end());
+      row_off_all.push_back(off);
+      cp.row_start = base;  // index into row_off_all; row_pair index recorded separately below
+      ctx->cps.push_back(cp);
+      ctx->ntasks += off;
+      // keep row_pair aligned with row_off (one padding entry for the trailing total)
+      row_pair_all.push_back(-1);
+    }
+  }
+  if ((rc = dev_upload(&ctx->d_row_pair, row_pair_all, own, s))) return rc;
+  if ((rc = dev_upload(&ctx->d_row_off, row_off_all, own, s))) return rc;
+  if ((rc = dev_upload(&ctx->d_col_pair, col_all, own, s))) return rc;
+  {
+    std::vector<unsigned long long> cz(kCountStride * std::max<size_t>(1, ctx->cps.size()), 0ull);
+    if ((rc = dev_upload(&ctx->d_counters, cz, own, s))) return rc;
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+  *out = guard.release();
+  return ASUKA_OK;
+}
+
+int asuka_jk_ctx_destroy(asuka_jk_ctx* ctx) {
+  if (!ctx) return ASUKA_OK;
+  (void)hipDeviceSynchronize();
+  delete ctx;
+  return ASUKA_OK;
+}
+
+int asuka_jk_ctx_info(const asuka_jk_ctx* ctx, int64_t out[8]) {
+  if (!ctx || !out) return fail(ASUKA_E_INVALID, "ctx and out must be non-null");
+  out[0] = ctx->nao;
+  out[1] = ctx->nshell;
+  out[2] = ctx->nsp;
+  out[3] = ctx->ntasks;
+  out[4] = ctx->max_l;
+  out[5] = (int64_t)ctx->cps.size();
+  out[6] = ctx->natm;
+  out[7] = (int64_t)ctx->pp.size();
+  return ASUKA_OK;
+}
+
+int asuka_jk_shell_pairs(const asuka_jk_ctx* ctx, int32_t* sp_A, int32_t* sp_B, double* Q) {
+  if (!ctx) return fail(ASUKA_E_INVALID, "ctx must be non-null");
+  for (int i = 0; i < ctx->nsp; ++i) {
+    if (sp_A) sp_A[i] = ctx->sp_A[i];
+    if (sp_B) sp_B[i] = ctx->sp_B[i];
+    if (Q) Q[i] = ctx->Q[i];
+  }
+  return ASUKA_OK;
+}
+
+int asuka_jk_class_stats(const asuka_jk_ctx* ctx, int64_t* rows, int cap) {
+  if (!ctx) return fail(ASUKA_E_INVALID, "ctx must be non-null");
+  int n = 0;
+  for (const auto& cp : ctx->cps) {
+    if (n >= cap) break;
+    int64_t* r = rows + 8 * n;
+    r[0] = cp.la;
+    r[1] = cp.lb;
+    r[2] = cp.lc;
+    r[3] = cp.ld;
+    r[4] = cp.nwork;
+    r[5] = cp.nprim_quartets;
+    r[6] = cp.nprim_roots;
+    r[7] = 0;
+    ++n;
+  }
+  return n;
+}
+
+}  // extern "C"
+
+static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int index) {
+  ClassWork w{};
+  w.cp_index = index;
+  w.la = cp.la;
+  w.lb = cp.lb;
+  w.lc = cp.lc;
+  w.ld = cp.ld;
+  w.row_pair = ctx->d_row_pair + cp.row_start;
+  w.row_off = ctx->d_row_off + cp.row_start;
+  w.col_pair = ctx->d_col_pair + cp.col_start;
+  w.nrows = cp.nrows;
+  w.nwork = cp.nwork;
+  return w;
+}
+
+static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool timed) {
+  if (timed) {
+    if (!ctx->ev_start) HIP_CHECK(hipEventCreate(&ctx->ev_start));
+    if (!ctx->ev_stop) HIP_CHECK(hipEventCreate(&ctx->ev_stop));
+    HIP_CHECK(hipEventRecord(ctx->ev_start, s));
+  }
+  long long n = 0;
+  for (size_t i = 0; i < ctx->cps.size(); ++i) {
+    ClassWork w = work_of(ctx, ctx->cps[i], (int)i);
+    bool handled = false;
+    HIP_CHECK(launch_eri_fast(ctx->dev, w, o, s, &handled));
+    if (!handled) HIP_CHECK(launch_eri_generic(ctx->dev, w, o, s));
+    ++n;
+  }
+  if (timed) HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
+  ctx->last_launches = n;
+  return ASUKA_OK;
+}
+
+static inline unsigned grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  return (unsigned)std::max<long long>(1, std::min<long long>(g, 65535));
+}
+
+extern "C" {
+
+int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream) {
+  if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  hipStream_t s = (hipStream_t)stream;
+  const bool wantJ = (flags & ASUKA_JK_WANT_J) && J;
+  const bool wantK = (flags & ASUKA_JK_WANT_K) && K;
+  if (!wantJ && !wantK) return ASUKA_OK;
+  const long long nn = (long long)ctx->nao * ctx->nao;
+  if (wantJ) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, J, nn, 0.0);
+  if (wantK) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, K, nn, 0.0);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, ctx->dev.sp_A, ctx->dev.sp_B,
+                     ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
+  HIP_CHECK(hipGetLastError());
+  EriOut o{};
+  o.mode = MODE_JK;
+  o.D = D;
+  o.J = wantJ ? J : nullptr;
+  o.K = wantK ? K : nullptr;
+  o.eps_density = ctx->eps_density;
+  if (flags & ASUKA_JK_COUNT) {
+    const long long nc = kCountStride * (long long)std::max<size_t>(1, ctx->cps.size());
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(nc, 256)), dim3(256), 0, s, ctx->d_counters, nc);
+    o.counters = ctx->d_counters;
+  }
+  int rc = run_classes(ctx, o, s, (flags & ASUKA_JK_TIME) != 0);
+  if (rc) return rc;
+  const int n = ctx->nao;
+  dim3 blk(16, 16), grd((n + 15) / 16, (n + 15) / 16);
+  if (wantJ) hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, s, J, n);
+  if (wantK) hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, s, K, n);
+  HIP_CHECK(hipGetLastError());
+  return ASUKA_OK;
+}
+
+int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, double* F, int flags, void* stream) {
+  if (!ctx || !D || !hcore || !F) return fail(ASUKA_E_INVALID, "ctx, D, hcore and F must be non-null");
+  if (F == D) return fail(ASUKA_E_INVALID, "F must not alias D");
+  hipStream_t s = (hipStream_t)stream;
+  const long long nn = (long long)ctx->nao * ctx->nao;
+  if (F != hcore) hipLaunchKernelGGL(k_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, F, hcore, nn);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, ctx->dev.sp_A, ctx->dev.sp_B,
+                     ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
+  HIP_CHECK(hipGetLastError());
+  EriOut o{};
+  o.mode = MODE_FOCK;
+  o.D = D;
+  o.F = F;
+  o.eps_density = ctx->eps_density;
+  if (flags & ASUKA_JK_COUNT) {
+    const long long nc = kCountStride * (long long)std::max<size_t>(1, ctx->cps.size());
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(nc, 256)), dim3(256), 0, s, ctx->d_counters, nc);
+    o.counters = ctx->d_counters;
+  }
+  int rc = run_classes(ctx, o, s, (flags & ASUKA_JK_TIME) != 0);
+  if (rc) return rc;
+  const int n = ctx->nao;
+  dim3 blk(16, 16), grd((n + 15) / 16, (n + 15) / 16);
+  hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, s, F, n);
+  HIP_CHECK(hipGetLastError());
+  return ASUKA_OK;
+}
+
+int asuka_jk_last_timing(asuka_jk_ctx* ctx, double* ms, int64_t* nlaunch) {
+  if (!ctx || !ms) return fail(ASUKA_E_INVALID, "ctx and ms must be non-null");
+  if (!ctx->ev_start || !ctx->ev_stop) return fail(ASUKA_E_INVALID, "no timed build recorded (use ASUKA_JK_TIME)");
+  HIP_CHECK(hipEventSynchronize(ctx->ev_stop));
+  float f = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&f, ctx->ev_start, ctx->ev_stop));
+  *ms = (double)f;
+  if (nlaunch) *nlaunch = ctx->last_launches;
+  return ASUKA_OK;
+}
+
+int asuka_jk_last_counts(asuka_jk_ctx* ctx, int64_t* rows, int cap, void* stream) {
+  if (!ctx || (cap > 0 && !rows)) return fail(ASUKA_E_INVALID, "ctx and rows must be non-null");
+  const int n = std::min<int>(cap, (int)ctx->cps.size());
+  if (n <= 0) return 0;
+  std::vector<unsigned long long> h(kCountStride * (size_t)n);
+  HIP_CHECK(hipMemcpyAsync(h.data(), ctx->d_counters, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
+                           (hipStream_t)stream));
+  HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  for (size_t i = 0; i < h.size(); ++i) rows[i] = (int64_t)h[i];
+  return n;
+}
+
+int asuka_jk_eri_tiles(asuka_jk_ctx* ctx, int64_t nq, const int32_t* spab, const int32_t* spcd, const int64_t* out_off,
+                       double* out, int flags, void* stream) {
+  if (!ctx || nq < 0 || (nq > 0 && (!spab || !spcd || !out_off || !out)))
+    return fail(ASUKA_E_INVALID, "tile arguments must be non-null");
+  (void)flags;
+  if (nq == 0) return ASUKA_OK;
+  ClassWork w{};
+  w.la = w.lb = w.lc = w.ld = ctx->max_l;
+  w.nwork = nq;
+  w.q_ab = spab;
+  w.q_cd = spcd;
+  w.q_out_off = (const long long*)out_off;
+  EriOut o{};
+  o.mode = MODE_TILE;
+  o.tile = out;
+  HIP_CHECK(launch_eri_generic(ctx->dev, w, o, (hipStream_t)stream));
+  return ASUKA_OK;
+}
+
+int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream) {
+  if (!ctx) return fail(ASUKA_E_INVALID, "ctx must be non-null");
+  if (V && ctx->natm <= 0) return fail(ASUKA_E_INVALID, "V needs atoms (natm > 0)");
+  HIP_CHECK(launch_int1e(ctx, S, T, V, (hipStream_t)stream));
+  return ASUKA_OK;
+}
+
+int asuka_rys_roots(int n, double T, double* x, double* w) {
+  if (n < 1 || n > RYS_NMAX) return fail(ASUKA_E_INVALID, "n must be in [1, 7]");
+  if (!(T >= 0.0)) return fail(ASUKA_E_INVALID, "T must be >= 0");
+  if (!x || !w) return fail(ASUKA_E_INVALID, "x and w must be non-null");
+  for (int u = 0; u < n; ++u) rys_root_host(n, u, T, x[u], w[u]);
+  return ASUKA_OK;
+}
+
+int asuka_rys_roots_device(int n, int64_t npts, const double* T, double* x, double* w, void* stream) {
+  if (n < 1 || n > RYS_NMAX) return fail(ASUKA_E_INVALID, "n must be in [1, 7]");
+  if (npts < 0 || (npts > 0 && (!T || !x || !w))) return fail(ASUKA_E_INVALID, "arrays must be non-null");
+  if (npts == 0) return ASUKA_OK;
+  hipLaunchKernelGGL(k_rys_points, dim3((unsigned)((npts + 127) / 128)), dim3(128), 0, (hipStream_t)stream, n,
+                     (long long)npts, T, x, w);
+  HIP_CHECK(hipGetLastError());
+  return ASUKA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// One-electron integrals, one thread per shell pair.
+// S and T: 1-D overlap recursion and the kinetic-from-overlap relation
+// (asuka/integrals/int1e_cart.py:77-115, :758-787). V: Rys quadrature with a
+// point-charge ket (the q -> infinity limit of the ERI recursion):
+// G[a+1] = (PA - x PC) G[a] + a (1-x)/(2p) G[a-1], HRR on the bra, weight
+// -Z 2 pi / p exp(-mu AB^2) sum_u w_u Ix Iy Iz, T = p |PC|^2; equal to the
+// reference's McMurchie-Davidson form (int1e_cart.py:1031-1190).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int L1 = ASUKA_LMAX;
+constexpr int LT = L1 + 3;  // overlap table needs lb + 2
+
+__device__ void overlap_1d(int la, int lb, double a, double b, double A, double B, double S[L1 + 1][LT]) {
+  const double p = a + b, ip = 1.0 / p, mu = a * b * ip;
+  const double Px = (a * A + b * B) * ip, PA = Px - A, PB = Px - B, AB = A - B;
+  for (int i = 0; i <= L1; ++i)
+    for (int j = 0; j < LT; ++j) S[i][j] = 0.0;
+  S[0][0] = sqrt(3.14159265358979323846 * ip) * exp(-mu * AB * AB);
+  const double i2p = 0.5 * ip;
+  for (int j = 0; j < lb; ++j) S[0][j + 1] = PB * S[0][j] + (j > 0 ? j * i2p * S[0][j - 1] : 0.0);
+  for (int i = 0; i < la; ++i) {
+    S[i + 1][0] = PA * S[i][0] + (i > 0 ? i * i2p * S[i - 1][0] : 0.0);
+    for (int j = 0; j < lb; ++j)
+      S[i + 1][j + 1] = PA * S[i][j + 1] + (i > 0 ? i * i2p * S[i - 1][j + 1] : 0.0) + (j + 1) * i2p * S[i][j];
+  }
+}
+
+__device__ double kin1d(double S[L1 + 1][LT], int i, int j, double b) {
+  double v = b * (2.0 * j + 1.0) * S[i][j] - 2.0 * b * b * S[i][j + 2];
+  if (j >= 2) v -= 0.5 * (double)(j * (j - 1)) * S[i][j - 2];
+  return v;
+}
+
+// 1-D nuclear-attraction factor for one root: I(i, j), i <= la, j <= lb.
+__device__ double nuc1d(int i, int j, double PA, double PC, double AB, double x, double i2p) {
+  double G[2 * L1 + 1];
+  const double c0 = PA - x * PC, b1 = (1.0 - x) * i2p;
+  G[0] = 1.0;
+  const int n = i + j;
+  if (n > 0) G[1] = c0;
+  for (int a = 1; a < n; ++a) G[a + 1] = c0 * G[a] + a * b1 * G[a - 1];
+  // (i, j) = sum_k C(j,k) AB^(j-k) G[i+k]
+  double out = 0.0, binom = 1.0;
+  for (int k = 0; k <= j; ++k) {
+    double pw = 1.0;
+    for (int e = 0; e < j - k; ++e) pw *= AB;
+    out += binom * pw * G[i + k];
+    binom = binom * (double)(j - k) / (double)(k + 1);
+  }
+  return out;
+}
+
+__global__ void k_int1e(const DevBasis b, const int* pstart, const int* pnum, const double* pexp, const double* pcoef,
+                        int natm, const double* axyz, const double* aZ, double* S, double* T, double* V) {
+  const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sp >= b.nsp) return;
+  const int A = b.sp_A[sp], B = b.sp_B[sp];
+  const int la = b.shell_l[A], lb = b.shell_l[B];
+  const int nA = ncart_c(la), nB = ncart_c(lb);
+  int ax[10], ay[10], az[10], bx[10], by[10], bz[10];
+  {
+    int i = 0;
+    for (int x = la; x >= 0; --x)
+      for (int y = la - x; y >= 0; --y) {
+        ax[i] = x;
+        ay[i] = y;
+        az[i] = la - x - y;
+        ++i;
+      }
+    i = 0;
+    for (int x = lb; x >= 0; --x)
+      for (int y = lb - x; y >= 0; --y) {
+        bx[i] = x;
+        by[i] = y;
+        bz[i] = lb - x - y;
+        ++i;
+      }
+  }
+  double s[100], t[100], v[100];
+  for (int i = 0; i < nA * nB; ++i) s[i] = t[i] = v[i] = 0.0;
+  const double* cA = b.shell_xyz + 3 * A;
+  const double* cB = b.shell_xyz + 3 * B;
+  const double ABx = cA[0] - cB[0], ABy = cA[1] - cB[1], ABz = cA[2] - cB[2];
+  const double AB2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  const int nr = (la + lb) / 2 + 1;
+  for (int ia_p = 0; ia_p < pnum[A]; ++ia_p)
+    for (int ib_p = 0; ib_p < pnum[B]; ++ib_p) {
+      const double a = pexp[pstart[A] + ia_p], bb = pexp[pstart[B] + ib_p];
+      const double c = pcoef[pstart[A] + ia_p] * pcoef[pstart[B] + ib_p];
+      if (c == 0.0) continue;
+      double Sx[L1 + 1][LT], Sy[L1 + 1][LT], Sz[L1 + 1][LT];
+      overlap_1d(la, lb + 2, a, bb, cA[0], cB[0], Sx);
+      overlap_1d(la, lb + 2, a, bb, cA[1], cB[1], Sy);
+      overlap_1d(la, lb + 2, a, bb, cA[2], cB[2], Sz);
+      for (int i = 0; i < nA; ++i)
+        for (int j = 0; j < nB; ++j) {
+          const double sx = Sx[ax[i]][bx[j]], sy = Sy[ay[i]][by[j]], sz = Sz[az[i]][bz[j]];
+          s[i * nB + j] += c * sx * sy * sz;
+          t[i * nB + j] += c * (kin1d(Sx, ax[i], bx[j], bb) * sy * sz + sx * kin1d(Sy, ay[i], by[j], bb) * sz +
+                                sx * sy * kin1d(Sz, az[i], bz[j], bb));
+        }
+      if (!V) continue;
+      const double p = a + bb, ip = 1.0 / p;
+      const double Px = (a * cA[0] + bb * cB[0]) * ip, Py = (a * cA[1] + bb * cB[1]) * ip,
+                   Pz = (a * cA[2] + bb * cB[2]) * ip;
+      const double Kab = exp(-a * bb * ip * AB2);
+      for (int ic = 0; ic < natm; ++ic) {
+        const double Z = aZ[ic];
+        if (Z == 0.0) continue;
+        const double PCx = Px - axyz[3 * ic], PCy = Py - axyz[3 * ic + 1], PCz = Pz - axyz[3 * ic + 2];
+        const double Targ = p * (PCx * PCx + PCy * PCy + PCz * PCz);
+        const double pref = -Z * 2.0 * 3.14159265358979323846 * ip * Kab * c;
+        for (int u = 0; u < nr; ++u) {
+          double xr, wr;
+          rys_root_dev(nr, u, Targ, xr, wr);
+          for (int i = 0; i < nA; ++i)
+            for (int j = 0; j < nB; ++j) {
+              const double Ix = nuc1d(ax[i], bx[j], Px - cA[0], PCx, ABx, xr, 0.5 * ip);
+              const double Iy = nuc1d(ay[i], by[j], Py - cA[1], PCy, ABy, xr, 0.5 * ip);
+              const double Iz = nuc1d(az[i], bz[j], Pz - cA[2], PCz, ABz, xr, 0.5 * ip);
+              v[i * nB + j] += pref * wr * Ix * Iy * Iz;
+            }
+        }
+      }
+    }
+  const long long N = b.nao;
+  const int a0 = b.shell_ao[A], b0 = b.shell_ao[B];
+  for (int i = 0; i < nA; ++i)
+    for (int j = 0; j < nB; ++j) {
+      const long long r = a0 + i, cc = b0 + j;
+      if (S) S[r * N + cc] = S[cc * N + r] = s[i * nB + j];
+      if (T) T[r * N + cc] = T[cc * N + r] = t[i * nB + j];
+      if (V) V[r * N + cc] = V[cc * N + r] = v[i * nB + j];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_int1e(const asuka_jk_ctx* ctx, double* S, double* T, double* V, hipStream_t s) {
+  hipLaunchKernelGGL(k_int1e, dim3((ctx->nsp + 63) / 64), dim3(64), 0, s, ctx->dev, ctx->d_prim_start,
+                     ctx->d_prim_num, ctx->d_prim_exp, ctx->d_prim_coef, ctx->natm, ctx->d_atom_xyz, ctx->d_atom_Z, S,
+                     T, V);
+  return hipGetLastError();
+}

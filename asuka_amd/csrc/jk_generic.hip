@@ -1,0 +1,256 @@
+// Generic wave-cooperative Rys ERI kernel with fused J/K (or Fock) digestion.
+//
+// One wavefront (64 lanes, one workgroup) owns one shell quartet at a time.
+// Per primitive quartet:
+//   lanes u < nroots      : Rys root/weight u from the interpolation table (LDS)
+//   lanes (u, axis) < 3n  : 2-D VRR table G[u][axis][i][k] (LDS), weight and
+//                           prefactor folded into the z axis
+//   all lanes             : 1-D HRR tables I[u][axis][ia][ib][ic][id] (LDS) via
+//                           the binomial transfer from G
+//   all lanes             : each lane accumulates its Cartesian components
+//                           sum_u Ix*Iy*Iz from LDS
+// After the primitive loop the lane's components are digested into J/K with the
+// ordered-contraction semantics of the reference
+// (cueri_cuda_kernels_dense_scatter.cu:246-352), folding the two symmetric
+// updates of each pair into one (the outputs are symmetrised afterwards).
+//
+// This kernel handles every class up to (ff|ff) with runtime angular momenta;
+// the class-specialised register kernels cover the hot low-l classes.
+#include "jk_internal.h"
+#include "rys_eval.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kNW = 16;  // components per lane per pass
+
+// Cartesian components of l = 0..3 concatenated, offset l(l+1)(l+2)/6.
+__constant__ signed char c_lx[20] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0, 3, 2, 2, 1, 1, 1, 0, 0, 0, 0};
+__constant__ signed char c_ly[20] = {0, 0, 1, 0, 0, 1, 0, 2, 1, 0, 0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
+__constant__ signed char c_lz[20] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2, 0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+__constant__ double c_binom[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
+
+__device__ inline int cart_off(int l) { return l * (l + 1) * (l + 2) / 6; }
+
+__device__ inline double sel3(int ax, double x, double y, double z) { return ax == 0 ? x : (ax == 1 ? y : z); }
+
+__device__ inline double ipow(double x, int k) {
+  double r = 1.0;
+  for (int i = 0; i < k; ++i) r *= x;
+  return r;
+}
+
+__global__ __launch_bounds__(kWave) void eri_generic_kernel(DevBasis b, ClassWork w, EriOut o) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x;
+  double qmax = 0.0;
+  int qmax_pair = -1;
+
+  for (long long k = blockIdx.x; k < w.nwork; k += gridDim.x) {
+    int P, Qp;
+    decode_work(w, k, P, Qp);
+    if (o.mode <= MODE_FOCK && o.eps_density > 0.0) {
+      const double bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
+      if (bound < o.eps_density) continue;  // uniform across the wave
+    }
+    const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
+    const int la = b.shell_l[A], lb = b.shell_l[B], lc = b.shell_l[C], ld = b.shell_l[Dsh];
+    const int nA = ncart_c(la), nB = ncart_c(lb), nC = ncart_c(lc), nD = ncart_c(ld);
+    const int nCD = nC * nD, ncomp = nA * nB * nCD;
+    const int nr = (la + lb + lc + ld) / 2 + 1;
+    const int nmax = la + lb, mmax = lc + ld, g1 = (nmax + 1) * (mmax + 1);
+    const int sIc = ld + 1, sIb = (lc + 1) * sIc, sIa = (lb + 1) * sIb, n1 = (la + 1) * sIa;
+
+    const double Ax = b.shell_xyz[3 * A], Ay = b.shell_xyz[3 * A + 1], Az = b.shell_xyz[3 * A + 2];
+    const double Bx = b.shell_xyz[3 * B], By = b.shell_xyz[3 * B + 1], Bz = b.shell_xyz[3 * B + 2];
+    const double Cx = b.shell_xyz[3 * C], Cy = b.shell_xyz[3 * C + 1], Cz = b.shell_xyz[3 * C + 2];
+    const double Dx = b.shell_xyz[3 * Dsh], Dy = b.shell_xyz[3 * Dsh + 1], Dz = b.shell_xyz[3 * Dsh + 2];
+    const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
+    const double CDx = Cx - Dx, CDy = Cy - Dy, CDz = Cz - Dz;
+
+    double* rw = lds;                          // [0, 7): roots, [7, 14): weights * prefactor
+    double* G = lds + 2 * ASUKA_NROOTS_MAX;    // nr * 3 * g1
+    double* I = G + nr * 3 * g1;               // nr * 3 * n1
+
+    const int ab0 = b.pp_start[P], nab = b.pp_count[P];
+    const int cd0 = b.pp_start[Qp], ncd = b.pp_count[Qp];
+    if (o.counters && lane == 0) {
+      unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
+      atomicAdd(cnt + (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp)), 1ull);
+      atomicAdd(cnt + 8, (unsigned long long)nab * (unsigned long long)ncd);
+    }
+    const int npass = (ncomp + kWave * kNW - 1) / (kWave * kNW);
+    const int oA = cart_off(la), oB = cart_off(lb), oC = cart_off(lc), oD = cart_off(ld);
+
+    for (int pass = 0; pass < npass; ++pass) {
+      int ix[kNW], iy[kNW], iz[kNW];
+      double acc[kNW];
+#pragma unroll
+      for (int j = 0; j < kNW; ++j) {
+        acc[j] = 0.0;
+        int comp = pass * kWave * kNW + j * kWave + lane;
+        if (comp >= ncomp) comp = 0;
+        const int iab = comp / nCD, icd = comp - iab * nCD;
+        const int ia = iab / nB, ib = iab - ia * nB, ic = icd / nD, id = icd - ic * nD;
+        ix[j] = c_lx[oA + ia] * sIa + c_lx[oB + ib] * sIb + c_lx[oC + ic] * sIc + c_lx[oD + id];
+        iy[j] = c_ly[oA + ia] * sIa + c_ly[oB + ib] * sIb + c_ly[oC + ic] * sIc + c_ly[oD + id];
+        iz[j] = c_lz[oA + ia] * sIa + c_lz[oB + ib] * sIb + c_lz[oC + ic] * sIc + c_lz[oD + id];
+      }
+
+      for (int ipq = 0; ipq < nab * ncd; ++ipq) {
+        const int iab_p = ipq / ncd, icd_p = ipq - iab_p * ncd;
+        const PrimPair X = b.pp[ab0 + iab_p];
+        const PrimPair Y = b.pp[cd0 + icd_p];
+        const double p = X.eta, q = Y.eta;
+        const double inv_pq = 1.0 / (p + q);
+        const double dPQx = X.Px - Y.Px, dPQy = X.Py - Y.Py, dPQz = X.Pz - Y.Pz;
+        const double T = p * q * inv_pq * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz);
+        if (lane < nr) {
+          double xr, wr;
+          rys_root_dev(nr, lane, T, xr, wr);
+          // 2 pi^(5/2) / (p q sqrt(p+q)) cK_ab cK_cd
+          const double base = 34.98683665524972497 * X.inv_eta * Y.inv_eta * sqrt(inv_pq) * X.cK * Y.cK;
+          rw[lane] = xr;
+          rw[ASUKA_NROOTS_MAX + lane] = wr * base;
+        }
+        __syncthreads();
+        if (lane < 3 * nr) {
+          const int u = lane / 3, ax = lane - 3 * u;
+          const double xr = rw[u];
+          const double B0 = xr * 0.5 * inv_pq;
+          const double B1 = (1.0 - xr) * 0.5 * X.inv_eta + B0;
+          const double B1p = (1.0 - xr) * 0.5 * Y.inv_eta + B0;
+          const double Pa = sel3(ax, X.Px, X.Py, X.Pz), Qa = sel3(ax, Y.Px, Y.Py, Y.Pz);
+          const double Cc = (Pa - sel3(ax, Ax, Ay, Az)) + (q * inv_pq) * xr * (Qa - Pa);
+          const double Cp = (Qa - sel3(ax, Cx, Cy, Cz)) + (p * inv_pq) * xr * (Pa - Qa);
+          double* g = G + (u * 3 + ax) * g1;
+          const int s = mmax + 1;
+          g[0] = (ax == 2) ? rw[ASUKA_NROOTS_MAX + u] : 1.0;
+          if (nmax > 0) g[s] = Cc * g[0];
+          if (mmax > 0) g[1] = Cp * g[0];
+          for (int a = 2; a <= nmax; ++a) g[a * s] = B1 * (double)(a - 1) * g[(a - 2) * s] + Cc * g[(a - 1) * s];
+          for (int c = 2; c <= mmax; ++c) g[c] = B1p * (double)(c - 1) * g[c - 2] + Cp * g[c - 1];
+          if (nmax > 0 && mmax > 0) {
+            for (int a = 1; a <= nmax; ++a) {
+              g[a * s + 1] = (double)a * B0 * g[(a - 1) * s] + Cp * g[a * s];
+              for (int c = 2; c <= mmax; ++c)
+                g[a * s + c] = B1p * (double)(c - 1) * g[a * s + c - 2] + (double)a * B0 * g[(a - 1) * s + c - 1] +
+                               Cp * g[a * s + c - 1];
+            }
+          }
+        }
+        __syncthreads();
+        for (int e = lane; e < nr * 3 * n1; e += kWave) {
+          const int u = e / (3 * n1);
+          const int rem = e - u * 3 * n1;
+          const int ax = rem / n1;
+          int t = rem - ax * n1;
+          const int ia_ = t / sIa;
+          t -= ia_ * sIa;
+          const int ib_ = t / sIb;
+          t -= ib_ * sIb;
+          const int ic_ = t / sIc;
+          const int id_ = t - ic_ * sIc;
+          const double ab = sel3(ax, ABx, ABy, ABz), cd = sel3(ax, CDx, CDy, CDz);
+          const double* g = G + (u * 3 + ax) * g1;
+          const int s = mmax + 1;
+          double val = 0.0;
+          for (int n = 0; n <= ib_; ++n) {
+            double inner = 0.0;
+            for (int m = 0; m <= id_; ++m) inner += c_binom[id_][m] * ipow(cd, id_ - m) * g[(ia_ + n) * s + ic_ + m];
+            val += c_binom[ib_][n] * ipow(ab, ib_ - n) * inner;
+          }
+          I[e] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kNW; ++j) {
+          double sacc = 0.0;
+          for (int u = 0; u < nr; ++u)
+            sacc += I[(u * 3) * n1 + ix[j]] * I[(u * 3 + 1) * n1 + iy[j]] * I[(u * 3 + 2) * n1 + iz[j]];
+          acc[j] += sacc;
+        }
+        __syncthreads();
+      }
+
+      // ---- digestion of this pass ----
+      const long long N = b.nao;
+      const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
+      const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
+#pragma unroll
+      for (int j = 0; j < kNW; ++j) {
+        const int comp = pass * kWave * kNW + j * kWave + lane;
+        if (comp >= ncomp) continue;
+        const double v = acc[j];
+        const int iab = comp / nCD, icd = comp - iab * nCD;
+        if (o.mode == MODE_TILE) {
+          o.tile[w.q_out_off[k] + comp] = v;
+          continue;
+        }
+        if (o.mode == MODE_SCHWARZ) {
+          if (iab == icd && v > qmax) {
+            qmax = v;
+            qmax_pair = P;
+          }
+          continue;
+        }
+        if (v == 0.0) continue;
+        const int ia = iab / nB, ib = iab - ia * nB, ic = icd / nD, id = icd - ic * nD;
+        const long long a = aoA + ia, bb = aoB + ib, c = aoC + ic, d = aoD + id;
+        const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0) * v;
+        const double fk = bk ? 2.0 * v : v;
+        if (o.mode == MODE_JK) {
+          if (o.J) {
+            atomicAdd(&o.J[a * N + bb], fj * o.D[c * N + d]);
+            if (bk) atomicAdd(&o.J[c * N + d], fj * o.D[a * N + bb]);
+          }
+          if (o.K) {
+            atomicAdd(&o.K[a * N + c], fk * o.D[bb * N + d]);
+            if (cd_neq) atomicAdd(&o.K[a * N + d], fk * o.D[bb * N + c]);
+            if (ab_neq) {
+              atomicAdd(&o.K[bb * N + c], fk * o.D[a * N + d]);
+              if (cd_neq) atomicAdd(&o.K[bb * N + d], fk * o.D[a * N + c]);
+            }
+          }
+        } else {  // MODE_FOCK: F += J - K/2
+          const double hk = -0.5 * fk;
+          atomicAdd(&o.F[a * N + bb], fj * o.D[c * N + d]);
+          if (bk) atomicAdd(&o.F[c * N + d], fj * o.D[a * N + bb]);
+          atomicAdd(&o.F[a * N + c], hk * o.D[bb * N + d]);
+          if (cd_neq) atomicAdd(&o.F[a * N + d], hk * o.D[bb * N + c]);
+          if (ab_neq) {
+            atomicAdd(&o.F[bb * N + c], hk * o.D[a * N + d]);
+            if (cd_neq) atomicAdd(&o.F[bb * N + d], hk * o.D[a * N + c]);
+          }
+        }
+      }
+    }
+    if (o.mode == MODE_SCHWARZ) {
+      // wave max over the lanes' diagonal elements of this quartet
+      double m = qmax;
+      for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+      if (lane == 0) o.Qout[P] = m > 0.0 ? sqrt(m) : 0.0;
+      qmax = 0.0;
+      qmax_pair = -1;
+    }
+  }
+  (void)qmax_pair;
+}
+
+}  // namespace
+
+size_t eri_generic_lds_bytes(int la, int lb, int lc, int ld) {
+  const int nr = (la + lb + lc + ld) / 2 + 1;
+  const int g1 = (la + lb + 1) * (lc + ld + 1);
+  const int n1 = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
+  return sizeof(double) * (2 * ASUKA_NROOTS_MAX + nr * 3 * g1 + nr * 3 * n1);
+}
+
+hipError_t launch_eri_generic(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {
+  if (w.nwork <= 0) return hipSuccess;
+  // LDS sized for the class (explicit lists may mix classes: then la..ld hold the maxima)
+  const size_t lds = eri_generic_lds_bytes(w.la, w.lb, w.lc, w.ld);
+  long long grid = w.nwork < 16384 ? w.nwork : 16384;
+  hipLaunchKernelGGL(eri_generic_kernel, dim3((unsigned)grid), dim3(kWave), lds, s, b, w, o);
+  return hipGetLastError();
+}

@@ -1,0 +1,103 @@
+// Internal types shared by the J/K translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#define ASUKA_LMAX 3  // highest shell angular momentum of this build (f)
+#define ASUKA_NCART_MAX 10
+#define ASUKA_NROOTS_MAX 7
+
+// One primitive pair of a shell pair (only pairs with a nonzero coefficient
+// product are kept; dropping exact zeros leaves every integral unchanged).
+struct __align__(16) PrimPair {
+  double eta;     // a + b
+  double Px, Py, Pz;
+  double cK;      // ca * cb * exp(-a b / eta |AB|^2)
+  double a_exp;   // exponent of the bra-side primitive (A)
+  double b_exp;   // exponent of the ket-side primitive (B)
+  double inv_eta;
+};
+
+enum EriMode : int {
+  MODE_JK = 0,       // accumulate J and/or K
+  MODE_FOCK = 1,     // accumulate J - K/2 into F
+  MODE_TILE = 2,     // write the tile
+  MODE_SCHWARZ = 3,  // Q[P] = sqrt(max diag (P|P))
+};
+
+// Device-side view of a context (passed by value to kernels).
+struct DevBasis {
+  const double* shell_xyz;   // (nshell, 3)
+  const int* shell_l;
+  const int* shell_ao;
+  const int* sp_A;
+  const int* sp_B;
+  const int* pp_start;       // (nsp) first primitive pair of the shell pair
+  const int* pp_count;       // (nsp)
+  const PrimPair* pp;
+  const double* Q;           // (nsp) Schwarz bounds
+  const double* Dsp;         // (nsp) max |D| per shell-pair block
+  int nao;
+  int nsp;
+};
+
+// Work of one class pair: rows are bra pairs (class X, Q-descending), each row
+// runs over a prefix of the ket list (class Y, Q-descending).
+struct ClassWork {
+  int la, lb, lc, ld;
+  const int* row_pair;        // (nrows)
+  const long long* row_off;   // (nrows + 1) exclusive prefix of row lengths
+  const int* col_pair;        // ket list
+  int nrows;
+  long long nwork;
+  // explicit quartet list (tile test hook); when set, rows/cols are unused
+  const int* q_ab;
+  const int* q_cd;
+  const long long* q_out_off;
+  int cp_index;               // class-pair index (statistics counters)
+};
+
+struct EriOut {
+  int mode;
+  const double* D;
+  double* J;  // MODE_JK (may be null)
+  double* K;  // MODE_JK (may be null)
+  double* F;  // MODE_FOCK
+  double* tile;
+  double* Qout;
+  double eps_density;  // <= 0 disables density screening
+  // optional statistics, kCountStride per class pair: [0..7] quartets evaluated
+  // by symmetry bin (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair), [8] their
+  // primitive quartets
+  unsigned long long* counters;
+};
+
+constexpr int kCountStride = 16;
+
+__host__ __device__ constexpr inline int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Decode work index k of a class pair into (bra pair, ket pair).
+__device__ inline void decode_work(const ClassWork& w, long long k, int& P, int& Qp) {
+  if (w.q_ab) {
+    P = w.q_ab[k];
+    Qp = w.q_cd[k];
+    return;
+  }
+  int lo = 0, hi = w.nrows - 1;
+  while (lo < hi) {  // largest r with row_off[r] <= k
+    int mid = (lo + hi + 1) >> 1;
+    if (w.row_off[mid] <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  P = w.row_pair[lo];
+  Qp = w.col_pair[(int)(k - w.row_off[lo])];
+}
+
+// Launchers (defined in jk_generic.hip / jk_fast.hip).
+hipError_t launch_eri_generic(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s);

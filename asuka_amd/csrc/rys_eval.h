@@ -1,0 +1,80 @@
+// Rys roots and weights from the generated interpolation table.
+//
+// The table data are included twice, once for device code (namespace rys_dev,
+// __device__ constant data) and once for host code (namespace rys_host), so the
+// same evaluator serves kernels and the host-side test hook.
+//
+// Roots are x = t^2 in [0, 1]; sum_u w_u x_u^k = F_k(T) for k < 2n.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rys_dev {
+#define RYS_TABLE_QUAL static const __device__
+#include "rys_table.h"
+#undef RYS_TABLE_QUAL
+}  // namespace rys_dev
+
+namespace rys_host {
+#define RYS_TABLE_QUAL static const
+#include "rys_table.h"
+#undef RYS_TABLE_QUAL
+}  // namespace rys_host
+
+// Evaluate root u (0-based, ascending) of the n-point rule at T.
+// Host and device variants differ only in which copy of the table they read.
+#define ASUKA_DEFINE_RYS_ONE(NAME, QUAL, NS)                                                  \
+  QUAL inline void NAME(int n, int u, double T, double& x, double& w) {                     \
+    if (T >= NS::rys_tmax[n]) {                                                             \
+      const double it = 1.0 / T;                                                            \
+      const int o = NS::rys_herm_off[n] + u;                                                \
+      x = NS::rys_herm_x[o] * it;                                                           \
+      w = NS::rys_herm_w[o] * sqrt(it);                                                     \
+      return;                                                                               \
+    }                                                                                       \
+    int i = (int)(T * RYS_INV_H);                                                           \
+    if (i >= NS::rys_nint[n]) i = NS::rys_nint[n] - 1;                                     \
+    const double s = (T - (double)i * RYS_H) * (2.0 * RYS_INV_H) - 1.0;                     \
+    const double* c = NS::rys_coef + NS::rys_coef_off[n] + (i * 2 * n + u) * (RYS_DEG + 1); \
+    const double* cw = c + n * (RYS_DEG + 1);                                               \
+    double ax = c[RYS_DEG], aw = cw[RYS_DEG];                                               \
+    _Pragma("unroll") for (int d = RYS_DEG - 1; d >= 0; --d) {                              \
+      ax = fma(ax, s, c[d]);                                                                \
+      aw = fma(aw, s, cw[d]);                                                               \
+    }                                                                                       \
+    x = ax;                                                                                 \
+    w = aw;                                                                                 \
+  }
+
+ASUKA_DEFINE_RYS_ONE(rys_root_dev, __device__, rys_dev)
+ASUKA_DEFINE_RYS_ONE(rys_root_host, , rys_host)
+
+// All n roots at once (compile-time n), device.
+template <int N>
+__device__ inline void rys_roots_dev(double T, double* x, double* w) {
+  if (T >= rys_dev::rys_tmax[N]) {
+    const double it = 1.0 / T;
+    const double sit = sqrt(it);
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      x[u] = rys_dev::rys_herm_x[rys_dev::rys_herm_off[N] + u] * it;
+      w[u] = rys_dev::rys_herm_w[rys_dev::rys_herm_off[N] + u] * sit;
+    }
+    return;
+  }
+  int i = (int)(T * RYS_INV_H);
+  if (i >= rys_dev::rys_nint[N]) i = rys_dev::rys_nint[N] - 1;
+  const double s = (T - (double)i * RYS_H) * (2.0 * RYS_INV_H) - 1.0;
+  const double* c = rys_dev::rys_coef + rys_dev::rys_coef_off[N] + i * 2 * N * (RYS_DEG + 1);
+#pragma unroll
+  for (int f = 0; f < 2 * N; ++f) {
+    const double* cf = c + f * (RYS_DEG + 1);
+    double a = cf[RYS_DEG];
+#pragma unroll
+    for (int d = RYS_DEG - 1; d >= 0; --d) a = fma(a, s, cf[d]);
+    if (f < N)
+      x[f] = a;
+    else
+      w[f - N] = a;
+  }
+}

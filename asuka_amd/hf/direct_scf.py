@@ -1,0 +1,259 @@
+"""RHF SCF driven by the direct Fock/J-K engine, on the GPU.
+
+Mirrors ``rhf_direct`` (asuka/hf/direct_scf.py:230-402) and its helpers
+(asuka/hf/df_scf.py:411-513 orthogonaliser, generalised eigensolver, RHF
+occupations, density; :665-670 commutator error; :722-835 DIIS):
+
+* core-Hamiltonian guess, then ``init_fock_cycles`` predictor cycles;
+* per cycle: F from ``direct_fock_rhf`` when max_l <= 2, else
+  h + J - K/2 from ``direct_JK`` (asuka/hf/direct_scf.py:331-335);
+* DIIS on e = FDS - (FDS)^T from ``diis_start_cycle``, with the reference's
+  Gram scaling, regularisation ladder and affine renormalisation, and a reset
+  when extrapolation fails;
+* eigh through X = S^(-1/2); D = 2 C_occ C_occ^T;
+* E = 1/2 sum D o (h + F) + E_nuc; converged when |dE| < conv_tol and
+  ||dD||_F < conv_tol_dm.
+
+Everything stays on the device except the few scalars the convergence test
+reads. Small eigenproblems (nao <= $ASUKA_HF_CPU_EIGH_MAX_NAO, default 96) are
+solved on the host as the reference does (asuka/hf/df_scf.py:27, :424-446).
+"""
+
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+
+from .direct_jk import DirectJKContext, direct_fock_rhf, direct_JK
+
+_CPU_EIGH_MAX_NAO = int(os.environ.get("ASUKA_HF_CPU_EIGH_MAX_NAO", "96") or 96)
+
+
+@dataclass(frozen=True)
+class SCFResult:
+    """asuka/hf/df_scf.py:838-848."""
+
+    method: str
+    converged: bool
+    niter: int
+    e_tot: float
+    e_elec: float
+    e_nuc: float
+    mo_energy: Any
+    mo_coeff: Any
+    mo_occ: Any
+    dm: Any = None
+
+
+def _sym(A):
+    return 0.5 * (A + A.T)
+
+
+def _orthogonalizer(torch, S, eps: float = 1e-12):
+    s, U = torch.linalg.eigh(_sym(S))
+    if bool((s <= eps).any()):
+        raise ValueError("S is not positive definite (small/negative eigenvalues)")
+    return (U * s.rsqrt()[None, :]) @ U.T
+
+
+def _gen_eigh(torch, F, X):
+    Fp = _sym(X.T @ _sym(F) @ X)
+    if Fp.shape[0] <= _CPU_EIGH_MAX_NAO:
+        e, Cp = np.linalg.eigh(Fp.cpu().numpy())
+        e = torch.as_tensor(e, device=F.device)
+        Cp = torch.as_tensor(Cp, device=F.device)
+    else:
+        e, Cp = torch.linalg.eigh(Fp)
+    return e, X @ Cp
+
+
+def _density(C, nocc: int):
+    Cw = C[:, :nocc] * (2.0 ** 0.5)
+    return _sym(Cw @ Cw.T)
+
+
+class _DIIS:
+    """asuka/hf/df_scf.py:722-835."""
+
+    def __init__(self, torch, max_vec: int = 8):
+        self.torch = torch
+        self.max_vec = int(max_vec)
+        self.F: list = []
+        self.e: list = []
+
+    def push(self, F, e):
+        self.F.append(F)
+        self.e.append(e)
+        if len(self.F) > self.max_vec:
+            self.F.pop(0)
+            self.e.pop(0)
+
+    def extrapolate(self):
+        torch = self.torch
+        n = len(self.F)
+        if n < 2:
+            return self.F[-1]
+        E = torch.stack([x.reshape(-1) for x in self.e])
+        G = (E @ E.T).cpu().numpy()
+        if not np.all(np.isfinite(G)):
+            return self.F[-1]
+        g = float(np.max(np.abs(G)))
+        G = G / (g if np.isfinite(g) and g > 0 else 1.0)
+        B = np.zeros((n + 1, n + 1))
+        B[:n, :n] = G
+        B[:n, n] = B[n, :n] = -1.0
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        coeff = None
+        ladder = (0.0, 1e-14, 1e-12, 1e-10, 1e-8, 1e-6)
+        for lam in ladder:
+            Br = B.copy()
+            Br[:n, :n] += lam * np.eye(n)
+            try:
+                c = np.linalg.solve(Br, rhs)
+            except np.linalg.LinAlgError:
+                continue
+            if np.all(np.isfinite(c)):
+                coeff = c
+                break
+        if coeff is None:
+            for lam in ladder[1:]:
+                Br = B.copy()
+                Br[:n, :n] += lam * np.eye(n)
+                c = np.linalg.lstsq(Br, rhs, rcond=None)[0]
+                if np.all(np.isfinite(c)):
+                    coeff = c
+                    break
+        if coeff is None:
+            return self.F[-1]
+        c = coeff[:n]
+        s = float(c.sum())
+        if not np.isfinite(s) or abs(s) < 1e-15:
+            return self.F[-1]
+        c = torch.as_tensor(c / s, device=self.F[-1].device, dtype=torch.float64)
+        out = torch.tensordot(c, torch.stack(self.F), dims=([0], [0]))
+        if not bool(torch.isfinite(out).all()):
+            return self.F[-1]
+        return out
+
+
+def rhf_direct(
+    S,
+    hcore,
+    jk_ctx: DirectJKContext,
+    *,
+    nelec: int,
+    enuc: float = 0.0,
+    max_cycle: int = 50,
+    conv_tol: float = 1e-10,
+    conv_tol_dm: float = 1e-8,
+    diis: bool = True,
+    diis_start_cycle: int = 1,
+    diis_space: int = 8,
+    damping: float = 0.0,
+    level_shift: float = 0.0,
+    dm0=None,
+    mo_coeff0=None,
+    init_fock_cycles: int = 1,
+    mixed_precision: bool | None = None,
+    profile: dict | None = None,
+) -> SCFResult:
+    """RHF with integral-direct J/K (asuka/hf/direct_scf.py:230-402)."""
+    import torch
+
+    dev = jk_ctx.device
+    S = torch.as_tensor(np.asarray(S) if not isinstance(S, torch.Tensor) else S).to(dev, torch.float64)
+    h = torch.as_tensor(np.asarray(hcore) if not isinstance(hcore, torch.Tensor) else hcore).to(dev, torch.float64)
+    nao = int(S.shape[0])
+    if tuple(S.shape) != (nao, nao) or tuple(h.shape) != (nao, nao):
+        raise ValueError("S/hcore must be (nao, nao)")
+    if nao != jk_ctx.nao:
+        raise ValueError(f"nao mismatch: S has {nao}, DirectJKContext has {jk_ctx.nao}")
+    nelec = int(nelec)
+    if nelec < 0 or nelec % 2:
+        raise ValueError("RHF requires an even nelec >= 0")
+    nocc = nelec // 2
+    occ = torch.zeros(nao, dtype=torch.float64, device=dev)
+    occ[:nocc] = 2.0
+    X = _orthogonalizer(torch, S)
+
+    use_fock = int(jk_ctx.max_l) <= 2
+
+    def fock(D):
+        if use_fock:
+            return direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed_precision)
+        J, K = direct_JK(jk_ctx, D, mixed_precision=mixed_precision)
+        return _sym(h + J - 0.5 * K)
+
+    def shifted(F):
+        if level_shift:
+            Fp = X.T @ F @ X + float(level_shift) * torch.eye(nao, dtype=torch.float64, device=dev)
+            return X @ Fp @ X.T
+        return F
+
+    eps, C = _gen_eigh(torch, h, X)
+    if dm0 is not None:
+        D = _sym(torch.as_tensor(np.asarray(dm0) if not isinstance(dm0, torch.Tensor) else dm0).to(dev, torch.float64))
+        if tuple(D.shape) != (nao, nao):
+            raise ValueError("dm0 must have shape (nao, nao)")
+    elif mo_coeff0 is not None:
+        C = torch.as_tensor(np.asarray(mo_coeff0) if not isinstance(mo_coeff0, torch.Tensor) else mo_coeff0).to(
+            dev, torch.float64)
+        if tuple(C.shape) != (nao, nao):
+            raise ValueError("mo_coeff0 must have shape (nao, nao)")
+        D = _density(C, nocc)
+    else:
+        D = _density(C, nocc)
+
+    lam = float(damping) if damping else 0.0
+    if init_fock_cycles > 0 and dm0 is None and mo_coeff0 is None:
+        for _ in range(int(init_fock_cycles)):
+            Dp = D
+            eps, C = _gen_eigh(torch, shifted(fock(Dp)), X)
+            D = _density(C, nocc)
+            if lam:
+                D = (1.0 - lam) * D + lam * Dp
+
+    diis_obj = _DIIS(torch, diis_space) if diis else None
+    e_last = None
+    converged = False
+    cycle = 0
+    njk = 0
+    for cycle in range(1, int(max_cycle) + 1):
+        F = shifted(fock(D))
+        njk += 1
+        if diis_obj is not None and cycle >= int(diis_start_cycle):
+            M = F @ D @ S
+            diis_obj.push(F, M - M.T)
+            try:
+                Ft = diis_obj.extrapolate()
+            except Exception:
+                Ft = None
+            if Ft is None or not bool(torch.isfinite(Ft).all()):
+                diis_obj = _DIIS(torch, diis_space)
+            else:
+                F = _sym(Ft)
+        eps, C = _gen_eigh(torch, F, X)
+        Dn = _density(C, nocc)
+        if lam:
+            Dn = (1.0 - lam) * Dn + lam * D
+        e_tot = 0.5 * float((Dn * (h + F)).sum().item()) + float(enuc)
+        dm_err = float(torch.linalg.norm(Dn - D).item())
+        de = abs(e_tot - e_last) if e_last is not None else float("inf")
+        D, e_last = Dn, e_tot
+        if de < conv_tol and dm_err < conv_tol_dm:
+            converged = True
+            break
+    if profile is not None:
+        profile.setdefault("scf", {})["iters"] = int(cycle)
+        profile["scf"]["jk_calls"] = njk
+    return SCFResult(method="RHF", converged=converged, niter=int(cycle),
+                     e_tot=float(e_last if e_last is not None else float("nan")),
+                     e_elec=float(e_last - float(enuc)) if e_last is not None else float("nan"),
+                     e_nuc=float(enuc), mo_energy=eps, mo_coeff=C, mo_occ=occ, dm=D)
+
+
+__all__ = ["SCFResult", "rhf_direct"]

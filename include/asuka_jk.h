@@ -1,0 +1,153 @@
+/*
+ * asuka_jk.h -- C ABI of the MI355X (gfx950) integral-direct four-center J/K engine.
+ *
+ * Plain C, no exceptions across the ABI, no torch or HIP types in the
+ * signatures: streams are passed as `void*` (a hipStream_t, NULL = default
+ * stream) and all matrices are caller-owned device pointers to row-major
+ * float64 (nao, nao) arrays. Every entry point returns 0 on success or a
+ * negative ASUKA_E_* code; asuka_jk_last_error() then holds a message
+ * (thread-local).
+ *
+ * Threading: one context per (device, basis). Calls on one context must be
+ * serialised by the caller; distinct contexts are independent.
+ *
+ * Reference interfaces replaced (asuka = zpengmei/ASUKA):
+ *   asuka/hf/direct_jk.py:1143  make_direct_jk_context  -> asuka_jk_ctx_create
+ *   asuka/hf/direct_jk.py:1530  direct_JK               -> asuka_jk_build
+ *   asuka/hf/direct_jk.py:2205  direct_fock_rhf         -> asuka_fock_build
+ *   asuka/cueri/screening.py:224 schwarz_shellpairs_device -> asuka_jk_schwarz
+ *   asuka/cueri/gpu.py:6512     eri_rys_generic_device (tiles) -> asuka_jk_eri_tiles
+ *   asuka/integrals/int1e_cart.py:1423 build_int1e_cart -> asuka_int1e
+ *   asuka/cueri/_eri_rys_cpu.pyx:2614 rys_roots_weights_cy -> asuka_rys_roots
+ * The binding layer replaced is the pybind11 module
+ * asuka/cueri/cuda/ext/src/cueri_cuda_ext.cpp:17-65 (caller-allocated outputs,
+ * stream-ordered, errors mapped to exceptions).
+ */
+#ifndef ASUKA_JK_H
+#define ASUKA_JK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASUKA_JK_ABI_VERSION 1
+
+/* status codes */
+#define ASUKA_OK 0
+#define ASUKA_E_INVALID (-1)      /* bad argument (maps to ValueError)           */
+#define ASUKA_E_UNSUPPORTED (-2)  /* angular momentum beyond the build (NotImplementedError) */
+#define ASUKA_E_HIP (-3)          /* HIP runtime error (RuntimeError)            */
+#define ASUKA_E_NOMEM (-4)        /* device allocation failed                     */
+
+/* build flags */
+#define ASUKA_JK_WANT_J 1
+#define ASUKA_JK_WANT_K 2
+#define ASUKA_JK_MIXED 4 /* per-quartet FP32 recursion below opts.tau_mixed */
+#define ASUKA_JK_COUNT 8 /* also count the quartets evaluated (asuka_jk_last_counts) */
+#define ASUKA_JK_TIME 16 /* record HIP events around the ERI kernels (asuka_jk_last_timing) */
+
+typedef struct asuka_jk_ctx asuka_jk_ctx;
+
+/* Packed Cartesian basis (host pointers), the SoA of asuka/cueri/basis_cart.py:11-77:
+ * one shell per contraction column, primitive coefficients with normalisation
+ * folded in, AO order = shell order with Cartesian components lx-desc, ly-desc. */
+typedef struct {
+  int32_t nshell;
+  const double* shell_cxyz;        /* (nshell, 3) Bohr */
+  const int32_t* shell_prim_start; /* (nshell) */
+  const int32_t* shell_nprim;      /* (nshell) */
+  const int32_t* shell_l;          /* (nshell) */
+  const int32_t* shell_ao_start;   /* (nshell) */
+  const int32_t* shell_atom;       /* (nshell) owning atom; may be NULL */
+  int32_t nprim;
+  const double* prim_exp;  /* (nprim) */
+  const double* prim_coef; /* (nprim) */
+  int32_t natm;
+  const double* atom_xyz;    /* (natm, 3) Bohr; may be NULL when natm == 0 */
+  const double* atom_charge; /* (natm); may be NULL when natm == 0 */
+} asuka_basis_desc;
+
+/* Context options (asuka/hf/direct_jk.py:1143-1150 keyword arguments). */
+typedef struct {
+  double eps_schwarz; /* keep quartets with Q_ab*Q_cd >= eps; 0 => Q := 1, no screening */
+  double eps_density; /* keep quartets with Q_ab*Q_cd*max(Dmax_ab, Dmax_cd) >= eps_density;
+                         < 0 => max(eps_schwarz, 1e-10) (asuka/hf/direct_jk.py:1594-1596); 0 disables */
+  double tau_mixed;   /* with ASUKA_JK_MIXED: quartets whose Q_ab*Q_cd*max|D| bound is below
+                         tau_mixed run the Rys recursion in FP32 */
+  int32_t reserved[8];
+} asuka_jk_opts;
+
+/* ABI version and last error message (thread-local). */
+int asuka_jk_version(void);
+const char* asuka_jk_last_error(void);
+
+/* Build a context: shell pairs, primitive-pair tables, Schwarz bounds (on the
+ * device), angular-momentum-class work lists. The context owns its device
+ * buffers. Replaces make_direct_jk_context (asuka/hf/direct_jk.py:1143-1470). */
+int asuka_jk_ctx_create(const asuka_basis_desc* basis, const asuka_jk_opts* opts, void* stream, asuka_jk_ctx** out);
+int asuka_jk_ctx_destroy(asuka_jk_ctx* ctx);
+
+/* Context facts: out[0]=nao, [1]=nshell, [2]=nsp, [3]=Schwarz-screened canonical
+ * quartets, [4]=max_l, [5]=number of class pairs with work, [6]=natm,
+ * [7]=primitive pairs kept (nonzero coefficient product). */
+int asuka_jk_ctx_info(const asuka_jk_ctx* ctx, int64_t out[8]);
+
+/* J and/or K for a symmetric density D. J and K are overwritten and returned
+ * symmetrised, ½(X + Xᵀ) (asuka/hf/direct_jk.py:2190-2195). Stream-ordered, no
+ * host synchronisation, no device allocation. Replaces direct_JK
+ * (asuka/hf/direct_jk.py:1530-2202). J or K may be NULL when not wanted. */
+int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream);
+
+/* F = h + J - ½K, symmetrised. F must not alias D. Replaces direct_fock_rhf
+ * (asuka/hf/direct_jk.py:2205-2713). */
+int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, double* F, int flags, void* stream);
+
+/* Shell pairs (l-ordered, asuka/cueri/shell_pairs.py:72-116) and their Schwarz
+ * bounds Q_ab = sqrt(max (ab|ab)) (asuka/cueri/screening.py:224-314), host copies. */
+int asuka_jk_shell_pairs(const asuka_jk_ctx* ctx, int32_t* sp_A, int32_t* sp_B, double* Q);
+
+/* Quartet statistics per class pair, for the throughput metrics. Fills up to
+ * `cap` rows of 8 int64: la, lb, lc, ld, schwarz-screened quartets, primitive
+ * quartets, sum over quartets of primitive-quartets x roots, 0. Returns the number
+ * of rows (>= 0) or an error code. */
+int asuka_jk_class_stats(const asuka_jk_ctx* ctx, int64_t* rows, int cap);
+
+/* Counters of the last build run with ASUKA_JK_COUNT, one row of 16 int64 per
+ * class pair in asuka_jk_class_stats order: [0..7] quartets evaluated after
+ * density screening, binned by (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair)
+ * (the symmetry histogram of asuka/hf/direct_jk_cost.py:134-294), [8] their
+ * primitive quartets, [9..15] 0. Synchronises the stream. Returns the number of rows. */
+int asuka_jk_last_counts(asuka_jk_ctx* ctx, int64_t* rows, int cap, void* stream);
+
+/* Milliseconds between the HIP events recorded (on the build's stream) before
+ * the first and after the last ERI/digestion kernel of the last build run with
+ * ASUKA_JK_TIME, and the number of those kernel launches. Synchronises on the
+ * end event. */
+int asuka_jk_last_timing(asuka_jk_ctx* ctx, double* ms, int64_t* nlaunch);
+
+/* Test hook: contracted Cartesian ERI tiles for an explicit quartet list
+ * (device arrays spab, spcd of length nq). Tile q is written at
+ * out + out_off[q] in (nAB, nCD) row-major order, with the shell-pair
+ * orientation of asuka_jk_shell_pairs (replaces the tile path of
+ * asuka/cueri/gpu.py:6512-6600). */
+int asuka_jk_eri_tiles(asuka_jk_ctx* ctx, int64_t nq, const int32_t* spab, const int32_t* spcd, const int64_t* out_off,
+                       double* out, int flags, void* stream);
+
+/* One-electron integrals S, T, V (nao x nao, device, any may be NULL)
+ * (asuka/integrals/int1e_cart.py:1423-1434). */
+int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream);
+
+/* Rys roots (x = t^2, ascending) and weights of the n-point rule at T, from the
+ * interpolation table the kernels use, evaluated on the host (n <= 7). */
+int asuka_rys_roots(int n, double T, double* x, double* w);
+
+/* Same table evaluated on the device for npts arguments: x, w are (npts, n). */
+int asuka_rys_roots_device(int n, int64_t npts, const double* T, double* x, double* w, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASUKA_JK_H */

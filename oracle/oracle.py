@@ -1,0 +1,420 @@
+"""CPU oracle for the direct J/K path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module, and only as the checker / the timed CPU
+baseline. The product package never imports it.
+
+Python side of the oracle. Each function restates the reference file:line named
+in its docstring; the arithmetic kernels live in ``eri_oracle.c`` and
+``int1e_oracle.c`` (built into ``liboracle.so`` by the Makefile here).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class _OracleBasis(ctypes.Structure):
+    _fields_ = [
+        ("nshell", ctypes.c_int),
+        ("shell_cxyz", _f64p),
+        ("shell_prim_start", _i32p),
+        ("shell_nprim", _i32p),
+        ("shell_l", _i32p),
+        ("shell_ao_start", _i32p),
+        ("prim_exp", _f64p),
+        ("prim_coef", _f64p),
+        ("nsp", ctypes.c_int),
+        ("sp_A", _i32p),
+        ("sp_B", _i32p),
+        ("sp_pair_start", _i32p),
+        ("pair_eta", _f64p),
+        ("pair_Px", _f64p),
+        ("pair_Py", _f64p),
+        ("pair_Pz", _f64p),
+        ("pair_cK", _f64p),
+    ]
+
+
+def build() -> str:
+    """Compile liboracle.so with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.oracle_boys_list.argtypes = [ctypes.c_double, ctypes.c_int, _f64p]
+        L.oracle_rys.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _f64p, _f64p]
+        L.oracle_pair_tables.argtypes = [ctypes.POINTER(_OracleBasis), _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.oracle_eri_tile.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
+        L.oracle_eri_tile.restype = ctypes.c_int
+        L.oracle_schwarz.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, ctypes.c_int, _f64p]
+        L.oracle_direct_jk.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_long, _i32p, _i32p, ctypes.c_int,
+                                       _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_int]
+        L.oracle_dense_eri.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
+        L.oracle_int1e.argtypes = [ctypes.c_int, _f64p, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, ctypes.c_int,
+                                   _f64p, _f64p, ctypes.c_int, _f64p, _f64p, _f64p]
+        L.oracle_int1e.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# Boys / Rys
+# ---------------------------------------------------------------------------
+
+
+def boys_list(T: float, m_max: int) -> np.ndarray:
+    """F_0..F_mmax(T), _eri_rys_cpu.pyx:187-217."""
+    out = np.zeros(m_max + 1)
+    lib().oracle_boys_list(float(T), int(m_max), _p(out, _f64p))
+    return out
+
+
+def rys_roots_weights(n: int, T: float, hp: bool = True) -> tuple[np.ndarray, np.ndarray]:
+    """Rys roots x = t^2 and weights, _eri_rys_cpu.pyx:219-684 (binary128 for n >= 4 when hp)."""
+    r = np.zeros(n)
+    w = np.zeros(n)
+    lib().oracle_rys(int(n), float(T), int(bool(hp)), _p(r, _f64p), _p(w, _f64p))
+    return r, w
+
+
+# ---------------------------------------------------------------------------
+# Shell pairs, pair tables, Schwarz, tasks
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class OracleContext:
+    basis: object
+    sp_A: np.ndarray
+    sp_B: np.ndarray
+    sp_npair: np.ndarray
+    sp_pair_start: np.ndarray
+    pair_eta: np.ndarray
+    pair_Px: np.ndarray
+    pair_Py: np.ndarray
+    pair_Pz: np.ndarray
+    pair_cK: np.ndarray
+    struct: _OracleBasis = None
+    _keep: tuple = ()
+
+    @property
+    def nsp(self) -> int:
+        return int(self.sp_A.shape[0])
+
+    @property
+    def nao(self) -> int:
+        return int(self.basis.nao)
+
+
+def shell_pairs_l_order(basis):
+    """All unordered shell pairs (i >= j), oriented so l_A >= l_B
+    (cueri/shell_pairs.py:72-116)."""
+    L = np.asarray(basis.shell_l)
+    npr = np.asarray(basis.shell_nprim)
+    A, B, npair = [], [], []
+    for i in range(basis.nshell):
+        for j in range(i + 1):
+            a, b = (j, i) if L[i] < L[j] else (i, j)
+            A.append(a)
+            B.append(b)
+            npair.append(int(npr[a]) * int(npr[b]))
+    start = np.zeros(len(A) + 1, dtype=np.int32)
+    start[1:] = np.cumsum(npair)
+    return (np.asarray(A, np.int32), np.asarray(B, np.int32), np.asarray(npair, np.int32), start)
+
+
+def make_context(basis) -> OracleContext:
+    sp_A, sp_B, sp_npair, sp_start = shell_pairs_l_order(basis)
+    tot = int(sp_start[-1])
+    ctx = OracleContext(basis, sp_A, sp_B, sp_npair, sp_start, *(np.zeros(tot) for _ in range(5)))
+    b = basis
+    arrs = (np.ascontiguousarray(b.shell_cxyz), np.ascontiguousarray(b.shell_prim_start),
+            np.ascontiguousarray(b.shell_nprim), np.ascontiguousarray(b.shell_l),
+            np.ascontiguousarray(b.shell_ao_start), np.ascontiguousarray(b.prim_exp),
+            np.ascontiguousarray(b.prim_coef))
+    st = _OracleBasis(
+        int(b.nshell), _p(arrs[0], _f64p), _p(arrs[1], _i32p), _p(arrs[2], _i32p), _p(arrs[3], _i32p),
+        _p(arrs[4], _i32p), _p(arrs[5], _f64p), _p(arrs[6], _f64p), ctx.nsp, _p(sp_A, _i32p), _p(sp_B, _i32p),
+        _p(sp_start, _i32p), _p(ctx.pair_eta, _f64p), _p(ctx.pair_Px, _f64p), _p(ctx.pair_Py, _f64p),
+        _p(ctx.pair_Pz, _f64p), _p(ctx.pair_cK, _f64p))
+    ctx.struct = st
+    ctx._keep = arrs
+    lib().oracle_pair_tables(ctypes.byref(st), _p(ctx.pair_eta, _f64p), _p(ctx.pair_Px, _f64p),
+                             _p(ctx.pair_Py, _f64p), _p(ctx.pair_Pz, _f64p), _p(ctx.pair_cK, _f64p))
+    return ctx
+
+
+def eri_tile(ctx: OracleContext, spAB: int, spCD: int, hp: bool = True) -> np.ndarray:
+    """Contracted Cartesian tile (nAB, nCD), _eri_rys_cpu.pyx:1096-1291."""
+    L = ctx.basis.shell_l
+    la, lb = int(L[ctx.sp_A[spAB]]), int(L[ctx.sp_B[spAB]])
+    lc, ld = int(L[ctx.sp_A[spCD]]), int(L[ctx.sp_B[spCD]])
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    out = np.zeros((nc(la) * nc(lb), nc(lc) * nc(ld)))
+    rc = lib().oracle_eri_tile(ctypes.byref(ctx.struct), int(spAB), int(spCD), int(bool(hp)), _p(out, _f64p))
+    if rc != 0:
+        raise ValueError("oracle tile: l too large")
+    return out
+
+
+def schwarz(ctx: OracleContext, hp: bool = True, threads: int = 0) -> np.ndarray:
+    """Q_AB = sqrt(max diag (AB|AB)), _eri_rys_cpu.pyx:2486-2611."""
+    Q = np.zeros(ctx.nsp)
+    lib().oracle_schwarz(ctypes.byref(ctx.struct), int(bool(hp)), int(threads), _p(Q, _f64p))
+    return Q
+
+
+def tasks_screened_sorted_q(Q: np.ndarray, eps: float) -> tuple[np.ndarray, np.ndarray]:
+    """Canonical screened quartets (spCD <= spAB, Q_ab Q_cd >= eps)
+    (cueri/tasks.py:49-115 ``build_tasks_screened_sorted_q``)."""
+    Q = np.asarray(Q, dtype=np.float64).ravel()
+    perm = np.argsort(-Q, kind="stable")
+    Qs = Q[perm]
+    n_valid = int(np.searchsorted(-Qs, 0.0, side="left"))
+    if n_valid == 0:
+        return np.empty(0, np.int32), np.empty(0, np.int32)
+    Qv = Qs[:n_valid]
+    if eps > 0:
+        thr = float(eps) / np.maximum(Qv, 1e-300)
+        jmax = np.searchsorted(-Qv, -thr, side="right").astype(np.int64)
+        jmax = np.minimum(jmax, np.arange(n_valid, dtype=np.int64) + 1)
+    else:
+        jmax = np.arange(1, n_valid + 1, dtype=np.int64)
+    i_arr = np.repeat(np.arange(n_valid, dtype=np.int64), jmax)
+    offs = np.zeros(n_valid + 1, dtype=np.int64)
+    np.cumsum(jmax, out=offs[1:])
+    j_arr = np.arange(int(offs[-1]), dtype=np.int64) - offs[i_arr]
+    p32 = perm[:n_valid].astype(np.int32)
+    ab, cd = p32[i_arr], p32[j_arr]
+    sw = cd > ab
+    ab2 = np.where(sw, cd, ab).astype(np.int32)
+    cd2 = np.where(sw, ab, cd).astype(np.int32)
+    return ab2, cd2
+
+
+def shell_pair_dmax(ctx: OracleContext, D: np.ndarray) -> np.ndarray:
+    """max |D_mu nu| over each shell-pair block, both orientations
+    (hf/direct_jk.py:1473-1527 ``_compute_D_sp_max``)."""
+    b = ctx.basis
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    out = np.zeros(ctx.nsp)
+    for sp in range(ctx.nsp):
+        A, B = int(ctx.sp_A[sp]), int(ctx.sp_B[sp])
+        a0, b0 = int(b.shell_ao_start[A]), int(b.shell_ao_start[B])
+        na, nb = nc(int(b.shell_l[A])), nc(int(b.shell_l[B]))
+        blk = np.abs(D[a0:a0 + na, b0:b0 + nb]).max()
+        if A != B:
+            blk = max(blk, np.abs(D[b0:b0 + nb, a0:a0 + na]).max())
+        out[sp] = blk
+    return out
+
+
+def density_screen(ctx, Q, D, ab, cd, eps_density):
+    """CSAM mask Q_ab Q_cd max(Dsp_ab, Dsp_cd) >= eps_density (hf/direct_jk.py:1893-1909)."""
+    dsp = shell_pair_dmax(ctx, D)
+    val = Q[ab] * Q[cd] * np.maximum(dsp[ab], dsp[cd])
+    keep = val >= eps_density
+    return ab[keep], cd[keep]
+
+
+def direct_jk_tasks(ctx: OracleContext, D: np.ndarray, ab: np.ndarray, cd: np.ndarray, hp: bool = True,
+                    threads: int = 0, want_J: bool = True, want_K: bool = True):
+    """Direct J/K over an explicit quartet list with the digestion semantics of
+    cueri_cuda_kernels_dense_scatter.cu:246-352; symmetrised as
+    hf/direct_jk.py:2190-2195."""
+    nao = ctx.nao
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    J = np.zeros((nao, nao)) if want_J else None
+    K = np.zeros((nao, nao)) if want_K else None
+    ab = np.ascontiguousarray(ab, dtype=np.int32)
+    cd = np.ascontiguousarray(cd, dtype=np.int32)
+    lib().oracle_direct_jk(ctypes.byref(ctx.struct), int(ab.size), _p(ab, _i32p), _p(cd, _i32p), nao,
+                           _p(D, _f64p), _p(J, _f64p), _p(K, _f64p), int(bool(hp)), int(threads))
+    if J is not None:
+        J = 0.5 * (J + J.T)
+    if K is not None:
+        K = 0.5 * (K + K.T)
+    return J, K
+
+
+def direct_jk(ctx: OracleContext, D: np.ndarray, eps_schwarz: float = 1e-12, eps_density: float | None = None,
+              Q: np.ndarray | None = None, hp: bool = True, threads: int = 0):
+    """Reference-semantics direct J/K: Schwarz-screened task list
+    (hf/direct_jk.py:1217-1246) then density screening at
+    eps_density = max(eps_schwarz, 1e-10) (hf/direct_jk.py:1594-1604)."""
+    if Q is None:
+        Q = schwarz(ctx, hp=hp, threads=threads) if eps_schwarz > 0 else np.ones(ctx.nsp)
+    ab, cd = tasks_screened_sorted_q(Q, eps_schwarz)
+    if eps_density is None:
+        eps_density = max(eps_schwarz, 1e-10)
+    if eps_density > 0:
+        ab, cd = density_screen(ctx, Q, D, ab, cd, eps_density)
+    return direct_jk_tasks(ctx, D, ab, cd, hp=hp, threads=threads)
+
+
+def dense_eri(ctx: OracleContext, hp: bool = True, threads: int = 0) -> np.ndarray:
+    """eri_mat[pq, rs] (nao^2 x nao^2), cueri/dense_cpu.py:663-733 semantics."""
+    nao = ctx.nao
+    eri = np.zeros((nao * nao, nao * nao))
+    lib().oracle_dense_eri(ctypes.byref(ctx.struct), nao, int(bool(hp)), int(threads), _p(eri, _f64p))
+    return eri
+
+
+def dense_JK(eri_mat: np.ndarray, D: np.ndarray):
+    """hf/dense_jk.py:55-81: J = einsum(mnls,ls), K = einsum(mlns,ls), symmetrised."""
+    nao = D.shape[0]
+    e4 = eri_mat.reshape(nao, nao, nao, nao)
+    J = np.einsum("mnls,ls->mn", e4, D, optimize=True)
+    K = np.einsum("mlns,ls->mn", e4, D, optimize=True)
+    return 0.5 * (J + J.T), 0.5 * (K + K.T)
+
+
+# ---------------------------------------------------------------------------
+# One-electron integrals and the dense RHF
+# ---------------------------------------------------------------------------
+
+
+def int1e(basis, coords_bohr: np.ndarray, charges: np.ndarray):
+    """(S, T, V), integrals/int1e_cart.py:1423-1434."""
+    b = basis
+    nao = int(b.nao)
+    S, T, V = (np.zeros((nao, nao)) for _ in range(3))
+    xyz = np.ascontiguousarray(coords_bohr, dtype=np.float64)
+    Z = np.ascontiguousarray(charges, dtype=np.float64)
+    arrs = [np.ascontiguousarray(a) for a in (b.shell_cxyz, b.shell_prim_start, b.shell_nprim, b.shell_l,
+                                               b.shell_ao_start, b.prim_exp, b.prim_coef)]
+    rc = lib().oracle_int1e(int(b.nshell), _p(arrs[0], _f64p), _p(arrs[1], _i32p), _p(arrs[2], _i32p),
+                            _p(arrs[3], _i32p), _p(arrs[4], _i32p), _p(arrs[5], _f64p), _p(arrs[6], _f64p),
+                            int(xyz.shape[0]), _p(xyz, _f64p), _p(Z, _f64p), nao, _p(S, _f64p), _p(T, _f64p),
+                            _p(V, _f64p))
+    if rc != 0:
+        raise ValueError("oracle int1e: l too large")
+    return S, T, V
+
+
+class _DIIS:
+    """Pulay DIIS, hf/df_scf.py:722-835 (Gram matrix scaled by its abs max,
+    regularisation ladder, affine renormalisation)."""
+
+    def __init__(self, max_vec: int = 8):
+        self.max_vec = max_vec
+        self.F, self.e = [], []
+
+    def push(self, F, e):
+        self.F.append(F)
+        self.e.append(e)
+        if len(self.F) > self.max_vec:
+            self.F.pop(0)
+            self.e.pop(0)
+
+    def extrapolate(self):
+        n = len(self.F)
+        if n < 2:
+            return self.F[-1]
+        E = np.stack([x.ravel() for x in self.e])
+        G = E @ E.T
+        g = float(np.max(np.abs(G)))
+        G = G / (g if g > 0 and np.isfinite(g) else 1.0)
+        B = np.zeros((n + 1, n + 1))
+        B[:n, :n] = G
+        B[:n, n] = B[n, :n] = -1.0
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        coeff = None
+        for lam in (0.0, 1e-14, 1e-12, 1e-10, 1e-8, 1e-6):
+            Br = B.copy()
+            Br[:n, :n] += lam * np.eye(n)
+            try:
+                c = np.linalg.solve(Br, rhs)
+            except np.linalg.LinAlgError:
+                continue
+            if np.all(np.isfinite(c)):
+                coeff = c
+                break
+        if coeff is None:
+            return self.F[-1]
+        c = coeff[:n]
+        s = c.sum()
+        if not np.isfinite(s) or abs(s) < 1e-15:
+            return self.F[-1]
+        c = c / s
+        return np.tensordot(c, np.stack(self.F), axes=(0, 0))
+
+
+def rhf(S, hcore, fock_fn, nelec: int, enuc: float = 0.0, max_cycle: int = 50, conv_tol: float = 1e-10,
+        conv_tol_dm: float = 1e-8, diis_space: int = 8, init_fock_cycles: int = 1, dm0=None):
+    """RHF loop of hf/direct_scf.py:230-402 with helpers hf/df_scf.py:411-513,
+    :665-670; fock_fn(D) -> F = sym(h + J - K/2)."""
+    sym = lambda A: 0.5 * (A + A.T)
+    s, U = np.linalg.eigh(sym(S))
+    X = U @ np.diag(s ** -0.5) @ U.T
+    nocc = nelec // 2
+
+    def gen_eigh(F):
+        e, Cp = np.linalg.eigh(sym(X.T @ sym(F) @ X))
+        return e, X @ Cp
+
+    def density(C):
+        Cw = C[:, :nocc] * np.sqrt(2.0)
+        return sym(Cw @ Cw.T)
+
+    eps, C = gen_eigh(hcore)
+    D = sym(np.asarray(dm0)) if dm0 is not None else density(C)
+    if dm0 is None:
+        for _ in range(init_fock_cycles):
+            eps, C = gen_eigh(fock_fn(D))
+            D = density(C)
+    diis = _DIIS(diis_space)
+    e_last = None
+    conv = False
+    cycle = 0
+    F = None
+    for cycle in range(1, max_cycle + 1):
+        F = fock_fn(D)
+        M = F @ D @ S
+        diis.push(F, M - M.T)
+        Ft = diis.extrapolate()
+        if Ft is None or not np.all(np.isfinite(Ft)):
+            diis = _DIIS(diis_space)
+        else:
+            F = sym(Ft)
+        eps, C = gen_eigh(F)
+        Dn = density(C)
+        e_tot = 0.5 * float(np.sum(Dn * (hcore + F))) + enuc
+        dm_err = float(np.linalg.norm(Dn - D))
+        de = abs(e_tot - e_last) if e_last is not None else float("inf")
+        D, e_last = Dn, e_tot
+        if de < conv_tol and dm_err < conv_tol_dm:
+            conv = True
+            break
+    return {"e_tot": e_last, "converged": conv, "niter": cycle, "mo_energy": eps, "mo_coeff": C, "dm": D}
+
+
+__all__ = [
+    "boys_list", "build", "dense_JK", "dense_eri", "density_screen", "direct_jk", "direct_jk_tasks", "eri_tile",
+    "int1e", "lib", "make_context", "rhf", "rys_roots_weights", "schwarz", "shell_pair_dmax", "shell_pairs_l_order",
+    "tasks_screened_sorted_q",
+]
