@@ -46,8 +46,9 @@ int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
 struct ClassPairHost {
   int X, Y;  // bra class >= ket class
   int la, lb, lc, ld;
-  int row_start, nrows, col_start;
-  long long nwork;
+  int seg_start, nseg;
+  long long nwork;      // padded work index range
+  long long nquartets;  // Schwarz-screened quartets
   long long nprim_quartets;
   long long nprim_roots;
 };
@@ -134,6 +135,7 @@ struct asuka_jk_ctx {
   int* d_row_pair = nullptr;
   long long* d_row_off = nullptr;
   int* d_col_pair = nullptr;
+  Segment* d_segs = nullptr;
   double* d_Q = nullptr;
   double* d_Dsp = nullptr;
   double* d_atom_xyz = nullptr;
@@ -314,31 +316,49 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   }
   for (auto& v : cls_pairs)
     std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return ctx->Q[a] > ctx->Q[b]; });
-  // per-class prefix sums of primitive-pair counts (for statistics)
-  std::vector<std::vector<long long>> cls_kpre(ncls);
-  for (int c = 0; c < ncls; ++c) {
-    cls_kpre[c].assign(cls_pairs[c].size() + 1, 0);
-    for (size_t i = 0; i < cls_pairs[c].size(); ++i) cls_kpre[c][i + 1] = cls_kpre[c][i] + ctx->pp_count[cls_pairs[c][i]];
-  }
   std::vector<int> cls_la(ncls), cls_lb(ncls);
   for (int la = 0; la <= ASUKA_LMAX; ++la)
     for (int lb = 0; lb <= la; ++lb) {
       cls_la[pair_class(la, lb)] = la;
       cls_lb[pair_class(la, lb)] = lb;
     }
+  // Pairs of each class are grouped by contraction depth (primitive pairs per
+  // shell pair), keeping the Q-descending order inside a group and each pair's
+  // rank in the full class list. A (bra group, ket group) segment then has
+  // uniform primitive-loop lengths; per bra pair its kets are a prefix of the
+  // ket group: Q_ket >= eps / Q_bra and, within one class, rank_ket <= rank_bra
+  // (each unordered pair of shell pairs once, asuka/hf/direct_jk.py:1228-1246).
+  struct PairGroup {
+    int kcount;
+    std::vector<int> pairs;  // Q-descending
+    std::vector<int> ranks;  // position in the full class list
+    int col_start = 0;
+  };
+  std::vector<std::vector<PairGroup>> groups(ncls);
   std::vector<int> col_all, row_pair_all;
   std::vector<long long> row_off_all;
-  std::vector<int> col_start(ncls, 0);
+  std::vector<Segment> segs;
   for (int c = 0; c < ncls; ++c) {
-    col_start[c] = (int)col_all.size();
-    col_all.insert(col_all.end(), cls_pairs[c].begin(), cls_pairs[c].end());
+    std::vector<int> kvals;
+    for (int sp : cls_pairs[c]) kvals.push_back(ctx->pp_count[sp]);
+    std::sort(kvals.begin(), kvals.end());
+    kvals.erase(std::unique(kvals.begin(), kvals.end()), kvals.end());
+    for (int kv : kvals) {
+      PairGroup g;
+      g.kcount = kv;
+      for (size_t r = 0; r < cls_pairs[c].size(); ++r)
+        if (ctx->pp_count[cls_pairs[c][r]] == kv) {
+          g.pairs.push_back(cls_pairs[c][r]);
+          g.ranks.push_back((int)r);
+        }
+      g.col_start = (int)col_all.size();
+      col_all.insert(col_all.end(), g.pairs.begin(), g.pairs.end());
+      groups[c].push_back(std::move(g));
+    }
   }
   const double eps = ctx->eps_schwarz;
   for (int X = 0; X < ncls; ++X) {
     for (int Y = 0; Y <= X; ++Y) {
-      const auto& rows = cls_pairs[X];
-      const auto& cols = cls_pairs[Y];
-      if (rows.empty() || cols.empty()) continue;
       ClassPairHost cp{};
       cp.X = X;
       cp.Y = Y;
@@ -346,43 +366,61 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       cp.lb = cls_lb[X];
       cp.lc = cls_la[Y];
       cp.ld = cls_lb[Y];
-      cp.row_start = (int)row_pair_all.size();
-      cp.col_start = col_start[Y];
+      cp.seg_start = (int)segs.size();
       const int nroots = (cp.la + cp.lb + cp.lc + cp.ld) / 2 + 1;
-      long long off = 0;
-      std::vector<long long> offs;
-      for (size_t r = 0; r < rows.size(); ++r) {
-        const int P = rows[r];
-        long long len;
-        if (eps > 0.0) {
-          const double thr = eps / std::max(ctx->Q[P], 1e-300);
-          // number of ket pairs with Q >= thr (cols are Q-descending)
-          len = std::partition_point(cols.begin(), cols.end(), [&](int c) { return ctx->Q[c] >= thr; }) - cols.begin();
-        } else {
-          len = (long long)cols.size();
+      long long padded = 0;
+      for (const PairGroup& bg : groups[X]) {
+        for (const PairGroup& kg : groups[Y]) {
+          const auto& cols = kg.pairs;
+          long long off = 0;
+          std::vector<long long> offs;
+          std::vector<int> rowp;
+          for (size_t r = 0; r < bg.pairs.size(); ++r) {
+            const int P = bg.pairs[r];
+            long long len;
+            if (eps > 0.0) {
+              const double thr = eps / std::max(ctx->Q[P], 1e-300);
+              len = std::partition_point(cols.begin(), cols.end(), [&](int c) { return ctx->Q[c] >= thr; }) -
+                    cols.begin();
+            } else {
+              len = (long long)cols.size();
+            }
+            if (X == Y)
+              len = std::min<long long>(len, std::upper_bound(kg.ranks.begin(), kg.ranks.end(), bg.ranks[r]) -
+                                                 kg.ranks.begin());
+            if (len <= 0) continue;
+            rowp.push_back(P);
+            offs.push_back(off);
+            off += len;
+          }
+          if (off == 0) continue;
+          Segment sg{};
+          sg.off = padded;
+          sg.len = off;
+          sg.row_start = (int)row_off_all.size();
+          sg.nrows = (int)offs.size();
+          sg.col_start = kg.col_start;
+          sg.kab = bg.kcount;
+          sg.kcd = kg.kcount;
+          segs.push_back(sg);
+          row_pair_all.insert(row_pair_all.end(), rowp.begin(), rowp.end());
+          row_pair_all.push_back(-1);  // keeps row_pair aligned with row_off's trailing total
+          row_off_all.insert(row_off_all.end(), offs.begin(), offs.end());
+          row_off_all.push_back(off);
+          padded += (off + 63) / 64 * 64;
+          cp.nquartets += off;
+          cp.nprim_quartets += (long long)bg.kcount * kg.kcount * off;
         }
-        if (X == Y) len = std::min<long long>(len, (long long)r + 1);
-        if (len <= 0) continue;
-        row_pair_all.push_back(P);
-        offs.push_back(off);
-        off += len;
-        cp.nprim_quartets += (long long)ctx->pp_count[P] * cls_kpre[Y][len];
       }
-      if (off == 0) continue;
-      cp.nrows = (int)offs.size();
-      cp.nwork = off;
+      cp.nseg = (int)segs.size() - cp.seg_start;
+      if (cp.nseg == 0) continue;
+      cp.nwork = padded;
       cp.nprim_roots = cp.nprim_quartets * nroots;
-      // row offsets are stored per class pair with a trailing total
-      const int base = (int)row_off_all.size();
-      row_off_all.insert(row_off_all.end(), offs.begin(), offs.end());
-      row_off_all.push_back(off);
-      cp.row_start = base;  // index into row_off_all; row_pair index recorded separately below
       ctx->cps.push_back(cp);
-      ctx->ntasks += off;
-      // keep row_pair aligned with row_off (one padding entry for the trailing total)
-      row_pair_all.push_back(-1);
+      ctx->ntasks += cp.nquartets;
     }
   }
+  if ((rc = dev_upload(&ctx->d_segs, segs, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_pair, row_pair_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_off, row_off_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_col_pair, col_all, own, s))) return rc;
@@ -435,7 +473,7 @@ int asuka_jk_class_stats(const asuka_jk_ctx* ctx, int64_t* rows, int cap) {
     r[1] = cp.lb;
     r[2] = cp.lc;
     r[3] = cp.ld;
-    r[4] = cp.nwork;
+    r[4] = cp.nquartets;
     r[5] = cp.nprim_quartets;
     r[6] = cp.nprim_roots;
     r[7] = 0;
@@ -453,10 +491,11 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   w.lb = cp.lb;
   w.lc = cp.lc;
   w.ld = cp.ld;
-  w.row_pair = ctx->d_row_pair + cp.row_start;
-  w.row_off = ctx->d_row_off + cp.row_start;
-  w.col_pair = ctx->d_col_pair + cp.col_start;
-  w.nrows = cp.nrows;
+  w.seg = ctx->d_segs + cp.seg_start;
+  w.nseg = cp.nseg;
+  w.row_pair = ctx->d_row_pair;
+  w.row_off = ctx->d_row_off;
+  w.col_pair = ctx->d_col_pair;
   w.nwork = cp.nwork;
   return w;
 }

@@ -1,0 +1,508 @@
+// Class-specialised register kernels: one lane per shell quartet.
+//
+// For a class (la lb|lc ld) every lane loops over the primitive quartets of its
+// shell quartet and, per Rys root, builds the 2-D tables
+//   G_x[i][k], i <= la+lb, k <= lc+ld   (VRR, weight*prefactor folded into z)
+// and accumulates the contracted intermediates
+//   (e|f) = sum_prim sum_root G_x[ex][fx] G_y[ey][fy] G_z[ez][fz]
+// for e in cart(la..la+lb), f in cart(lc..lc+ld), all in registers. The
+// horizontal transfer to (ab|cd) runs once per shell quartet after the
+// primitive loop (its coefficients depend only on A-B and C-D):
+//   (e|cd) = sum_m prod_i C(d_i,m_i) CD_i^(d_i-m_i) (e|c+m)
+//   (ab|cd) = sum_n prod_i C(b_i,n_i) AB_i^(b_i-n_i) (a+n|cd)
+// The tile then feeds the J/K (or Fock) digestion of the reference's ordered
+// contraction (cueri_cuda_kernels_dense_scatter.cu:246-352) with block sums
+// per quartet: nAB + nCD J updates and up to 4 K blocks, instead of one atomic
+// per tile element and term. Symmetric pair updates are folded into one (the
+// outputs are symmetrised afterwards).
+//
+// All component indices are compile-time (constexpr tables + full unrolling),
+// so the tables and accumulators live in VGPRs.
+#pragma once
+#include "jk_internal.h"
+#include "rys_eval.h"
+
+namespace asuka_fast {
+
+constexpr int nc_(int l) { return (l + 1) * (l + 2) / 2; }
+
+// components of l in lx-descending, ly-descending order
+constexpr int cx_(int l, int i) {
+  int idx = 0;
+  for (int x = l; x >= 0; --x)
+    for (int y = l - x; y >= 0; --y) {
+      if (idx == i) return x;
+      ++idx;
+    }
+  return 0;
+}
+constexpr int cy_(int l, int i) {
+  int idx = 0;
+  for (int x = l; x >= 0; --x)
+    for (int y = l - x; y >= 0; --y) {
+      if (idx == i) return y;
+      ++idx;
+    }
+  return 0;
+}
+constexpr int cidx_(int lx, int ly, int lz) {
+  const int l = lx + ly + lz;
+  return (l - lx) * (l - lx + 1) / 2 + lz;
+}
+constexpr int binom_(int n, int k) {
+  int r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+// Concatenated Cartesian range l in [lo, hi]: index -> exponents and back.
+template <int LO, int HI>
+struct Range {
+  static constexpr int size() {
+    int s = 0;
+    for (int l = LO; l <= HI; ++l) s += nc_(l);
+    return s;
+  }
+  static constexpr int off(int l) {
+    int s = 0;
+    for (int k = LO; k < l; ++k) s += nc_(k);
+    return s;
+  }
+  struct Tab {
+    int x[size()], y[size()], z[size()];
+  };
+  static constexpr Tab tab() {
+    Tab t{};
+    int i = 0;
+    for (int l = LO; l <= HI; ++l)
+      for (int c = 0; c < nc_(l); ++c) {
+        t.x[i] = cx_(l, c);
+        t.y[i] = cy_(l, c);
+        t.z[i] = l - t.x[i] - t.y[i];
+        ++i;
+      }
+    return t;
+  }
+  static constexpr int index(int x, int y, int z) { return off(x + y + z) + cidx_(x, y, z); }
+};
+
+__device__ __forceinline__ double pw(const double* p, int k) { return p[k]; }
+
+template <int LA, int LB, int LC, int LD, int SA, int SC>
+struct Plan {
+  static constexpr int la = LA, lb = LB, lc = LC, ld = LD;
+  static constexpr int NA = nc_(LA), NB = nc_(LB), NC = nc_(LC), ND = nc_(LD);
+  static constexpr int NMAX = LA + LB, MMAX = LC + LD;
+  static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+  static constexpr int SPLIT_A = SA, SPLIT_C = SC, NCH = SA * SC;
+  using RE = Range<LA, LA + LB>;
+  using RF = Range<LC, LC + LD>;
+  static constexpr int NE = RE::size(), NF = RF::size();
+};
+
+// The chunk CH of a plan covers A-components [a0, a1) and C-components
+// [c0, c1); it needs the intermediates (e|f) for e = a + n (|n| <= lb) and
+// f = c + m (|m| <= ld) only.
+template <class PL, int CH>
+struct Chunk {
+  static constexpr int ca = CH / PL::SPLIT_C, cc = CH % PL::SPLIT_C;
+  static constexpr int a0 = ca * PL::NA / PL::SPLIT_A, a1 = (ca + 1) * PL::NA / PL::SPLIT_A;
+  static constexpr int c0 = cc * PL::NC / PL::SPLIT_C, c1 = (cc + 1) * PL::NC / PL::SPLIT_C;
+  struct Lists {
+    int e[PL::NE];
+    int emap[PL::NE];
+    int ne;
+    int f[PL::NF];
+    int fmap[PL::NF];
+    int nf;
+  };
+  static constexpr Lists lists() {
+    Lists L{};
+    bool ue[PL::NE] = {}, uf[PL::NF] = {};
+    for (int a = a0; a < a1; ++a) {
+      const int x = cx_(PL::la, a), y = cy_(PL::la, a), z = PL::la - x - y;
+      for (int nx = 0; nx <= PL::lb; ++nx)
+        for (int ny = 0; ny + nx <= PL::lb; ++ny)
+          for (int nz = 0; nz + ny + nx <= PL::lb; ++nz) ue[PL::RE::index(x + nx, y + ny, z + nz)] = true;
+    }
+    for (int c = c0; c < c1; ++c) {
+      const int x = cx_(PL::lc, c), y = cy_(PL::lc, c), z = PL::lc - x - y;
+      for (int mx = 0; mx <= PL::ld; ++mx)
+        for (int my = 0; my + mx <= PL::ld; ++my)
+          for (int mz = 0; mz + my + mx <= PL::ld; ++mz) uf[PL::RF::index(x + mx, y + my, z + mz)] = true;
+    }
+    L.ne = 0;
+    for (int i = 0; i < PL::NE; ++i) {
+      L.emap[i] = -1;
+      if (ue[i]) {
+        L.e[L.ne] = i;
+        L.emap[i] = L.ne++;
+      }
+    }
+    L.nf = 0;
+    for (int i = 0; i < PL::NF; ++i) {
+      L.fmap[i] = -1;
+      if (uf[i]) {
+        L.f[L.nf] = i;
+        L.fmap[i] = L.nf++;
+      }
+    }
+    return L;
+  }
+  static constexpr Lists L = lists();
+  static constexpr int NEL = L.ne, NFL = L.nf;
+};
+
+// Sum over the 64 lanes of the wavefront (all lanes must be converged).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <class PL, int CH>
+__device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, bool active) {
+  using CK = Chunk<PL, CH>;
+  constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
+  constexpr int NMAX = PL::NMAX, MMAX = PL::MMAX;
+  constexpr int LA = PL::la, LB = PL::lb, LC = PL::lc, LD = PL::ld;
+  constexpr int NB = PL::NB, ND = PL::ND;
+  constexpr int a0 = CK::a0, a1 = CK::a1, c0 = CK::c0, c1 = CK::c1;
+  constexpr int NAL = a1 - a0, NCL = c1 - c0;
+  constexpr int NCDL = NCL * ND;
+  constexpr auto te = PL::RE::tab();
+  constexpr auto tf = PL::RF::tab();
+  constexpr auto L = CK::L;
+
+  const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
+  const double Ax = b.shell_xyz[3 * A], Ay = b.shell_xyz[3 * A + 1], Az = b.shell_xyz[3 * A + 2];
+  const double Cx = b.shell_xyz[3 * C], Cy = b.shell_xyz[3 * C + 1], Cz = b.shell_xyz[3 * C + 2];
+  const int ab0 = b.pp_start[P], nab = active ? b.pp_count[P] : 0;
+  const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
+
+  double ef[NEL * NFL];
+#pragma unroll
+  for (int i = 0; i < NEL * NFL; ++i) ef[i] = 0.0;
+
+  for (int i = 0; i < nab; ++i) {
+    const PrimPair X = b.pp[ab0 + i];
+    const double PAx = X.Px - Ax, PAy = X.Py - Ay, PAz = X.Pz - Az;
+    for (int j = 0; j < ncd; ++j) {
+      const PrimPair Y = b.pp[cd0 + j];
+      const double p = X.eta, q = Y.eta;
+      const double inv_pq = 1.0 / (p + q);
+      const double dx = X.Px - Y.Px, dy = X.Py - Y.Py, dz = X.Pz - Y.Pz;
+      const double T = p * q * inv_pq * (dx * dx + dy * dy + dz * dz);
+      // 2 pi^(5/2) / (p q sqrt(p+q)) cK_ab cK_cd
+      const double base = 34.98683665524972497 * (X.cK * X.inv_eta) * (Y.cK * Y.inv_eta) * sqrt(inv_pq);
+      double rx[NR], rw[NR];
+      rys_roots_dev<NR>(T, rx, rw);
+      const double qf = q * inv_pq, pf = p * inv_pq;
+      const double QCx = Y.Px - Cx, QCy = Y.Py - Cy, QCz = Y.Pz - Cz;
+#pragma unroll
+      for (int u = 0; u < NR; ++u) {
+        const double x = rx[u];
+        const double B0 = 0.5 * x * inv_pq;
+        const double B1 = 0.5 * (1.0 - x) * X.inv_eta + B0;
+        const double B1p = 0.5 * (1.0 - x) * Y.inv_eta + B0;
+        const double xq = qf * x, xp = pf * x;
+        double G[3][NMAX + 1][MMAX + 1];
+        const double c00[3] = {PAx - xq * dx, PAy - xq * dy, PAz - xq * dz};
+        const double c0p[3] = {QCx + xp * dx, QCy + xp * dy, QCz + xp * dz};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          const double g0 = (ax == 2) ? rw[u] * base : 1.0;
+          const double Cc = c00[ax], Cp = c0p[ax];
+          G[ax][0][0] = g0;
+          if (NMAX > 0) G[ax][1][0] = Cc * g0;
+#pragma unroll
+          for (int a = 2; a <= NMAX; ++a) G[ax][a][0] = B1 * (a - 1) * G[ax][a - 2][0] + Cc * G[ax][a - 1][0];
+          if (MMAX > 0) {
+#pragma unroll
+            for (int a = 0; a <= NMAX; ++a) {
+              G[ax][a][1] = Cp * G[ax][a][0] + (a > 0 ? a * B0 * G[ax][a > 0 ? a - 1 : 0][0] : 0.0);
+#pragma unroll
+              for (int c = 2; c <= MMAX; ++c)
+                G[ax][a][c] = B1p * (c - 1) * G[ax][a][c - 2] + Cp * G[ax][a][c - 1] +
+                              (a > 0 ? a * B0 * G[ax][a > 0 ? a - 1 : 0][c - 1] : 0.0);
+            }
+          }
+        }
+#pragma unroll
+        for (int ie = 0; ie < NEL; ++ie) {
+#pragma unroll
+          for (int jf = 0; jf < NFL; ++jf) {
+            const int eg = L.e[ie], fg = L.f[jf];
+            const double gxy = G[0][te.x[eg]][tf.x[fg]] * G[1][te.y[eg]][tf.y[fg]];
+            ef[ie * NFL + jf] = fma(gxy, G[2][te.z[eg]][tf.z[fg]], ef[ie * NFL + jf]);
+          }
+        }
+      }
+    }
+  }
+
+  // ---- horizontal transfer: ket (C-components of the chunk) then bra ----
+  const double Bx = b.shell_xyz[3 * B], By = b.shell_xyz[3 * B + 1], Bz = b.shell_xyz[3 * B + 2];
+  const double Dx = b.shell_xyz[3 * Dsh], Dy = b.shell_xyz[3 * Dsh + 1], Dz = b.shell_xyz[3 * Dsh + 2];
+  double pab[3][LB + 1], pcd[3][LD + 1];
+  {
+    const double ab[3] = {Ax - Bx, Ay - By, Az - Bz};
+    const double cd[3] = {Cx - Dx, Cy - Dy, Cz - Dz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      pab[ax][0] = 1.0;
+      pcd[ax][0] = 1.0;
+#pragma unroll
+      for (int e = 1; e <= LB; ++e) pab[ax][e] = pab[ax][e - 1] * ab[ax];
+#pragma unroll
+      for (int e = 1; e <= LD; ++e) pcd[ax][e] = pcd[ax][e - 1] * cd[ax];
+    }
+  }
+  double ecd[NEL * NCDL];
+#pragma unroll
+  for (int ie = 0; ie < NEL; ++ie)
+#pragma unroll
+    for (int ic = 0; ic < NCL; ++ic)
+#pragma unroll
+      for (int id = 0; id < ND; ++id) {
+        const int cxv = cx_(LC, c0 + ic), cyv = cy_(LC, c0 + ic), czv = LC - cxv - cyv;
+        const int dxv = cx_(LD, id), dyv = cy_(LD, id), dzv = LD - dxv - dyv;
+        double s = 0.0;
+#pragma unroll
+        for (int mx = 0; mx <= dxv; ++mx)
+#pragma unroll
+          for (int my = 0; my <= dyv; ++my)
+#pragma unroll
+            for (int mz = 0; mz <= dzv; ++mz) {
+              const double coef = (double)(binom_(dxv, mx) * binom_(dyv, my) * binom_(dzv, mz)) *
+                                  pw(pcd[0], dxv - mx) * pw(pcd[1], dyv - my) * pw(pcd[2], dzv - mz);
+              s = fma(coef, ef[ie * NFL + L.fmap[PL::RF::index(cxv + mx, cyv + my, czv + mz)]], s);
+            }
+        ecd[ie * NCDL + ic * ND + id] = s;
+      }
+  double tile[NAL * NB * NCDL];
+#pragma unroll
+  for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      const int axv = cx_(LA, a0 + ia), ayv = cy_(LA, a0 + ia), azv = LA - axv - ayv;
+      const int bxv = cx_(LB, ib), byv = cy_(LB, ib), bzv = LB - bxv - byv;
+#pragma unroll
+      for (int jc = 0; jc < NCDL; ++jc) {
+        double s = 0.0;
+#pragma unroll
+        for (int nx = 0; nx <= bxv; ++nx)
+#pragma unroll
+          for (int ny = 0; ny <= byv; ++ny)
+#pragma unroll
+            for (int nz = 0; nz <= bzv; ++nz) {
+              const double coef = (double)(binom_(bxv, nx) * binom_(byv, ny) * binom_(bzv, nz)) *
+                                  pw(pab[0], bxv - nx) * pw(pab[1], byv - ny) * pw(pab[2], bzv - nz);
+              s = fma(coef, ecd[L.emap[PL::RE::index(axv + nx, ayv + ny, azv + nz)] * NCDL + jc], s);
+            }
+        tile[(ia * NB + ib) * NCDL + jc] = s;
+      }
+    }
+
+  // ---- digestion of the chunk's tile block (rows a0..a1 x all b, cols c0..c1 x all d) ----
+  // Sums over the chunk's own components are complete; the rest are partial
+  // and the atomics add them up across chunks.
+  const long long N = b.nao;
+  const int aoA = b.shell_ao[A] + a0, aoB = b.shell_ao[B], aoC = b.shell_ao[C] + c0, aoD = b.shell_ao[Dsh];
+  const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
+  const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
+  const double fk = bk ? 2.0 : 1.0;
+  const double* Dm = o.D;
+  double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
+  double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
+  const double ks = (o.mode == MODE_FOCK) ? -0.5 * fk : fk;
+  if (Jo) {
+    double dcd[NCDL];
+#pragma unroll
+    for (int ic = 0; ic < NCL; ++ic)
+#pragma unroll
+      for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = Dm[(aoC + ic) * N + aoD + id];
+    // The lanes of a row share the bra pair, hence the J[a,b] addresses: when
+    // the whole wave has one bra pair, reduce across lanes and issue one
+    // atomic per element instead of 64 contending ones.
+    const unsigned long long amask = __ballot(active);
+    const int first = amask ? (__ffsll((unsigned long long)amask) - 1) : 0;
+    const int Pfirst = __shfl(P, first);
+    const bool uniform = __all(!active || P == Pfirst);
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < NAL * NB; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < NCDL; ++c) s = fma(tile[r * NCDL + c], dcd[c], s);
+      s = active ? fj * s : 0.0;
+      if (uniform) {
+        s = wave_sum(s);
+        if (lane == first && active) atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
+      } else if (active) {
+        atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
+      }
+    }
+    if (bk && active) {
+      double dab[NAL * NB];
+#pragma unroll
+      for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = Dm[(aoA + ia) * N + aoB + ib];
+#pragma unroll
+      for (int c = 0; c < NCDL; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < NAL * NB; ++r) s = fma(tile[r * NCDL + c], dab[r], s);
+        atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
+      }
+    }
+  }
+  if (Ko && active) {
+    {  // K[a,c] += sum_{b,d} v D[b,d]
+      double dbd[NB * ND];
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = Dm[(aoB + ib) * N + aoD + id];
+#pragma unroll
+      for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+        for (int ic = 0; ic < NCL; ++ic) {
+          double s = 0.0;
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dbd[ib * ND + id], s);
+          atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], ks * s);
+        }
+    }
+    if (cd_neq) {  // K[a,d] += sum_{b,c} v D[b,c]
+      double dbc[NB * NCL];
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < NCL; ++ic) dbc[ib * NCL + ic] = Dm[(aoB + ib) * N + aoC + ic];
+#pragma unroll
+      for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) {
+          double s = 0.0;
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < NCL; ++ic) s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dbc[ib * NCL + ic], s);
+          atomicAdd(&Ko[(aoA + ia) * N + aoD + id], ks * s);
+        }
+    }
+    if (ab_neq) {  // K[b,c] += sum_{a,d} v D[a,d]
+      double dad[NAL * ND];
+#pragma unroll
+      for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) dad[ia * ND + id] = Dm[(aoA + ia) * N + aoD + id];
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < NCL; ++ic) {
+          double s = 0.0;
+#pragma unroll
+          for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dad[ia * ND + id], s);
+          atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], ks * s);
+        }
+      if (cd_neq) {  // K[b,d] += sum_{a,c} v D[a,c]
+        double dac[NAL * NCL];
+#pragma unroll
+        for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+          for (int ic = 0; ic < NCL; ++ic) dac[ia * NCL + ic] = Dm[(aoA + ia) * N + aoC + ic];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) {
+            double s = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+              for (int ic = 0; ic < NCL; ++ic)
+                s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dac[ia * NCL + ic], s);
+            atomicAdd(&Ko[(aoB + ib) * N + aoD + id], ks * s);
+          }
+      }
+    }
+  }
+}
+
+template <class PL, int CH>
+__device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp,
+                                               bool active) {
+  if constexpr (CH < PL::NCH) {
+    if (chunk == CH) {
+      run_chunk<PL, CH>(b, o, P, Qp, active);
+      return;
+    }
+    dispatch_chunk<PL, CH + 1>(chunk, b, o, P, Qp, active);
+  }
+}
+
+// Work mapping: wavefront g of the grid-stride loop takes 64 consecutive
+// quartets (one per lane) and one chunk, chunk = g mod NCH, so the chunk (and
+// with it all register indexing) is uniform across the wave.
+template <class PL>
+__global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o) {
+  constexpr int NCH = PL::NCH;
+  const long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long wave = gtid >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const long long ngroups = (w.nwork + 63) >> 6;
+  for (long long g = wave; g < ngroups * NCH; g += nwaves) {  // uniform trip count per wave
+    const int chunk = (int)(g % NCH);
+    const long long k = (g / NCH) * 64 + lane;
+    bool active = k < w.nwork;
+    int P = 0, Qp = 0;
+    if (active) active = decode_work(w, k, P, Qp);
+    if (active) {
+      if (o.eps_density > 0.0) {
+        const double bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
+        active = bound >= o.eps_density;
+      }
+    }
+    if (o.counters && chunk == 0 && active) {
+      const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
+      unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
+      atomicAdd(cnt + (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp)), 1ull);
+      atomicAdd(cnt + 8, (unsigned long long)b.pp_count[P] * (unsigned long long)b.pp_count[Qp]);
+    }
+    // inactive lanes (tail, screened) run with empty primitive loops and no
+    // atomics so the wave stays converged for the cross-lane reductions
+    dispatch_chunk<PL, 0>(chunk, b, o, P, Qp, active);
+  }
+}
+
+template <class PL>
+hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {
+  const int block = 256;  // 4 waves
+  const long long waves = ((w.nwork + 63) / 64) * PL::NCH;
+  long long g = (waves + 3) / 4;
+  if (g > 16384) g = 16384;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(block), 0, s, b, w, o);
+  return hipGetLastError();
+}
+
+using Launcher = hipError_t (*)(const DevBasis&, const ClassWork&, const EriOut&, hipStream_t);
+
+}  // namespace asuka_fast
+
+// Each jk_fast_*.hip translation unit instantiates a subset of class plans.
+#define ASUKA_FAST_FIND(FN, LIST)                                                     \
+  asuka_fast::Launcher FN(int la, int lb, int lc, int ld) {                          \
+    using namespace asuka_fast;                                                       \
+    LIST(ASUKA_FAST_CASE)                                                             \
+    return nullptr;                                                                   \
+  }
+#define ASUKA_FAST_CASE(a, b_, c, d, sa, sc) \
+  if (la == a && lb == b_ && lc == c && ld == d) return &launch_plan<Plan<a, b_, c, d, sa, sc>>;

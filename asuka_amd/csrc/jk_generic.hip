@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kWave) void eri_generic_kernel(DevBasis b, ClassWor
 
   for (long long k = blockIdx.x; k < w.nwork; k += gridDim.x) {
     int P, Qp;
-    decode_work(w, k, P, Qp);
+    if (!decode_work(w, k, P, Qp)) continue;  // segment padding (uniform across the wave)
     if (o.mode <= MODE_FOCK && o.eps_density > 0.0) {
       const double bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
       if (bound < o.eps_density) continue;  // uniform across the wave
@@ -71,6 +71,34 @@ __global__ __launch_bounds__(kWave) void eri_generic_kernel(DevBasis b, ClassWor
     double* rw = lds;                          // [0, 7): roots, [7, 14): weights * prefactor
     double* G = lds + 2 * ASUKA_NROOTS_MAX;    // nr * 3 * g1
     double* I = G + nr * 3 * g1;               // nr * 3 * n1
+    // J/K block accumulators and the density blocks they need (JK/Fock modes)
+    const int nAB = nA * nB;
+    double* Lj_ab = I + nr * 3 * n1;
+    double* Lj_cd = Lj_ab + nAB;
+    double* Lk_ac = Lj_cd + nCD;
+    double* Lk_ad = Lk_ac + nA * nC;
+    double* Lk_bc = Lk_ad + nA * nD;
+    double* Lk_bd = Lk_bc + nB * nC;
+    double* Dcd = Lk_bd + nB * nD;
+    double* Dab = Dcd + nCD;
+    double* Dbd = Dab + nAB;
+    double* Dbc = Dbd + nB * nD;
+    double* Dad = Dbc + nB * nC;
+    double* Dac = Dad + nA * nD;
+    const int nblk = nAB + nCD + nA * nC + nA * nD + nB * nC + nB * nD;
+    const bool digest = o.mode <= MODE_FOCK;
+    if (digest) {
+      const long long N = b.nao;
+      const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
+      for (int e = lane; e < nblk; e += kWave) Lj_ab[e] = 0.0;
+      for (int e = lane; e < nCD; e += kWave) Dcd[e] = o.D[(aoC + e / nD) * N + aoD + e % nD];
+      for (int e = lane; e < nAB; e += kWave) Dab[e] = o.D[(aoA + e / nB) * N + aoB + e % nB];
+      for (int e = lane; e < nB * nD; e += kWave) Dbd[e] = o.D[(aoB + e / nD) * N + aoD + e % nD];
+      for (int e = lane; e < nB * nC; e += kWave) Dbc[e] = o.D[(aoB + e / nC) * N + aoC + e % nC];
+      for (int e = lane; e < nA * nD; e += kWave) Dad[e] = o.D[(aoA + e / nD) * N + aoD + e % nD];
+      for (int e = lane; e < nA * nC; e += kWave) Dac[e] = o.D[(aoA + e / nC) * N + aoC + e % nC];
+      __syncthreads();
+    }
 
     const int ab0 = b.pp_start[P], nab = b.pp_count[P];
     const int cd0 = b.pp_start[Qp], ncd = b.pp_count[Qp];
@@ -173,9 +201,7 @@ __global__ __launch_bounds__(kWave) void eri_generic_kernel(DevBasis b, ClassWor
         __syncthreads();
       }
 
-      // ---- digestion of this pass ----
-      const long long N = b.nao;
-      const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
+      // ---- digestion of this pass: block sums in LDS ----
       const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
 #pragma unroll
       for (int j = 0; j < kNW; ++j) {
@@ -194,36 +220,46 @@ __global__ __launch_bounds__(kWave) void eri_generic_kernel(DevBasis b, ClassWor
           }
           continue;
         }
-        if (v == 0.0) continue;
         const int ia = iab / nB, ib = iab - ia * nB, ic = icd / nD, id = icd - ic * nD;
-        const long long a = aoA + ia, bb = aoB + ib, c = aoC + ic, d = aoD + id;
-        const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0) * v;
-        const double fk = bk ? 2.0 * v : v;
-        if (o.mode == MODE_JK) {
-          if (o.J) {
-            atomicAdd(&o.J[a * N + bb], fj * o.D[c * N + d]);
-            if (bk) atomicAdd(&o.J[c * N + d], fj * o.D[a * N + bb]);
-          }
-          if (o.K) {
-            atomicAdd(&o.K[a * N + c], fk * o.D[bb * N + d]);
-            if (cd_neq) atomicAdd(&o.K[a * N + d], fk * o.D[bb * N + c]);
-            if (ab_neq) {
-              atomicAdd(&o.K[bb * N + c], fk * o.D[a * N + d]);
-              if (cd_neq) atomicAdd(&o.K[bb * N + d], fk * o.D[a * N + c]);
-            }
-          }
-        } else {  // MODE_FOCK: F += J - K/2
-          const double hk = -0.5 * fk;
-          atomicAdd(&o.F[a * N + bb], fj * o.D[c * N + d]);
-          if (bk) atomicAdd(&o.F[c * N + d], fj * o.D[a * N + bb]);
-          atomicAdd(&o.F[a * N + c], hk * o.D[bb * N + d]);
-          if (cd_neq) atomicAdd(&o.F[a * N + d], hk * o.D[bb * N + c]);
-          if (ab_neq) {
-            atomicAdd(&o.F[bb * N + c], hk * o.D[a * N + d]);
-            if (cd_neq) atomicAdd(&o.F[bb * N + d], hk * o.D[a * N + c]);
-          }
+        atomicAdd(&Lj_ab[iab], v * Dcd[icd]);
+        if (bk) atomicAdd(&Lj_cd[icd], v * Dab[iab]);
+        atomicAdd(&Lk_ac[ia * nC + ic], v * Dbd[ib * nD + id]);
+        if (cd_neq) atomicAdd(&Lk_ad[ia * nD + id], v * Dbc[ib * nC + ic]);
+        if (ab_neq) {
+          atomicAdd(&Lk_bc[ib * nC + ic], v * Dad[ia * nD + id]);
+          if (cd_neq) atomicAdd(&Lk_bd[ib * nD + id], v * Dac[ia * nC + ic]);
         }
       }
+    }
+    if (digest) {
+      // flush the block sums: one global atomic per block entry
+      __syncthreads();
+      const long long N = b.nao;
+      const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
+      const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
+      const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
+      const double fk = (bk ? 2.0 : 1.0) * (o.mode == MODE_FOCK ? -0.5 : 1.0);
+      double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
+      double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
+      if (Jo) {
+        for (int e = lane; e < nAB; e += kWave) atomicAdd(&Jo[(aoA + e / nB) * N + aoB + e % nB], fj * Lj_ab[e]);
+        if (bk)
+          for (int e = lane; e < nCD; e += kWave) atomicAdd(&Jo[(aoC + e / nD) * N + aoD + e % nD], fj * Lj_cd[e]);
+      }
+      if (Ko) {
+        for (int e = lane; e < nA * nC; e += kWave) atomicAdd(&Ko[(aoA + e / nC) * N + aoC + e % nC], fk * Lk_ac[e]);
+        if (cd_neq)
+          for (int e = lane; e < nA * nD; e += kWave)
+            atomicAdd(&Ko[(aoA + e / nD) * N + aoD + e % nD], fk * Lk_ad[e]);
+        if (ab_neq) {
+          for (int e = lane; e < nB * nC; e += kWave)
+            atomicAdd(&Ko[(aoB + e / nC) * N + aoC + e % nC], fk * Lk_bc[e]);
+          if (cd_neq)
+            for (int e = lane; e < nB * nD; e += kWave)
+              atomicAdd(&Ko[(aoB + e / nD) * N + aoD + e % nD], fk * Lk_bd[e]);
+        }
+      }
+      __syncthreads();  // LDS is reused by the next quartet
     }
     if (o.mode == MODE_SCHWARZ) {
       // wave max over the lanes' diagonal elements of this quartet
@@ -243,7 +279,9 @@ size_t eri_generic_lds_bytes(int la, int lb, int lc, int ld) {
   const int nr = (la + lb + lc + ld) / 2 + 1;
   const int g1 = (la + lb + 1) * (lc + ld + 1);
   const int n1 = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
-  return sizeof(double) * (2 * ASUKA_NROOTS_MAX + nr * 3 * g1 + nr * 3 * n1);
+  const int nA = ncart_c(la), nB = ncart_c(lb), nC = ncart_c(lc), nD = ncart_c(ld);
+  const int nblk = nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD;
+  return sizeof(double) * (2 * ASUKA_NROOTS_MAX + nr * 3 * g1 + nr * 3 * n1 + 2 * nblk);
 }
 
 hipError_t launch_eri_generic(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {

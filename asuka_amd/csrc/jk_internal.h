@@ -45,20 +45,36 @@ struct DevBasis {
   int nsp;
 };
 
-// Work of one class pair: rows are bra pairs (class X, Q-descending), each row
-// runs over a prefix of the ket list (class Y, Q-descending).
+// Work of one class pair. Quartets are grouped into segments of equal bra and
+// ket contraction depth (primitive pairs per shell pair), so the quartets of a
+// segment share both primitive-loop lengths. In a segment, rows are bra pairs
+// (Q-descending) and each row runs over a prefix of the segment's ket list
+// (Q-descending). Segment starts are padded to multiples of 64 in the class
+// pair's work index space, so a wavefront never spans two segments; padding
+// indices decode as invalid.
+struct Segment {
+  long long off;   // first work index (multiple of 64)
+  long long len;   // quartets in the segment
+  int row_start;   // into row_pair / row_off (row_off has nrows + 1 entries)
+  int nrows;
+  int col_start;   // into col_pair
+  int kab, kcd;    // primitive pairs of every bra / ket pair in the segment
+  int pad;
+};
+
 struct ClassWork {
   int la, lb, lc, ld;
-  const int* row_pair;        // (nrows)
-  const long long* row_off;   // (nrows + 1) exclusive prefix of row lengths
-  const int* col_pair;        // ket list
-  int nrows;
-  long long nwork;
-  // explicit quartet list (tile test hook); when set, rows/cols are unused
+  const Segment* seg;
+  int nseg;
+  const int* row_pair;
+  const long long* row_off;
+  const int* col_pair;
+  long long nwork;  // padded work index range
+  // explicit quartet list (tile test hook); when set, segments are unused
   const int* q_ab;
   const int* q_cd;
   const long long* q_out_off;
-  int cp_index;               // class-pair index (statistics counters)
+  int cp_index;  // class-pair index (statistics counters)
 };
 
 struct EriOut {
@@ -80,23 +96,37 @@ constexpr int kCountStride = 16;
 
 __host__ __device__ constexpr inline int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
 
-// Decode work index k of a class pair into (bra pair, ket pair).
-__device__ inline void decode_work(const ClassWork& w, long long k, int& P, int& Qp) {
+// Decode work index k of a class pair into (bra pair, ket pair); false for
+// segment padding.
+__device__ inline bool decode_work(const ClassWork& w, long long k, int& P, int& Qp) {
   if (w.q_ab) {
     P = w.q_ab[k];
     Qp = w.q_cd[k];
-    return;
+    return true;
   }
-  int lo = 0, hi = w.nrows - 1;
-  while (lo < hi) {  // largest r with row_off[r] <= k
-    int mid = (lo + hi + 1) >> 1;
-    if (w.row_off[mid] <= k)
+  int lo = 0, hi = w.nseg - 1;
+  while (lo < hi) {  // largest segment with off <= k
+    const int mid = (lo + hi + 1) >> 1;
+    if (w.seg[mid].off <= k)
       lo = mid;
     else
       hi = mid - 1;
   }
-  P = w.row_pair[lo];
-  Qp = w.col_pair[(int)(k - w.row_off[lo])];
+  const Segment sg = w.seg[lo];
+  const long long kk = k - sg.off;
+  if (kk >= sg.len) return false;
+  const long long* ro = w.row_off + sg.row_start;
+  int a = 0, c = sg.nrows - 1;
+  while (a < c) {  // largest row with ro[r] <= kk
+    const int mid = (a + c + 1) >> 1;
+    if (ro[mid] <= kk)
+      a = mid;
+    else
+      c = mid - 1;
+  }
+  P = w.row_pair[sg.row_start + a];
+  Qp = w.col_pair[sg.col_start + (int)(kk - ro[a])];
+  return true;
 }
 
 // Launchers (defined in jk_generic.hip / jk_fast.hip).

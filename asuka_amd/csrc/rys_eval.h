@@ -76,5 +76,7 @@ __device__ inline void rys_roots_dev(double T, double* x, double* w) {
       x[f] = a;
     else
       w[f - N] = a;
+    // keep one function's coefficients live at a time (bounds VGPR use)
+    __builtin_amdgcn_sched_barrier(0);
   }
 }

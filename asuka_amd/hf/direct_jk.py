@@ -74,9 +74,12 @@ class DirectJKContext:
             self.handle = 0
 
     def class_stats(self) -> list[dict]:
-        """Per class pair: (la, lb, lc, ld), Schwarz-screened shell quartets,
-        primitive quartets and primitive-quartet roots."""
-        cap = 128
+        """Per work segment (class pair and ket contraction depth): (la, lb, lc,
+        ld), Schwarz-screened shell quartets, primitive quartets and
+        primitive-quartet roots."""
+        info = np.zeros(8, dtype=np.int64)
+        _lib.check(_lib.lib().asuka_jk_ctx_info(ctypes.c_void_p(self.handle), info.ctypes.data_as(_lib._i64p)))
+        cap = max(1, int(info[5]))
         rows = np.zeros((cap, 8), dtype=np.int64)
         n = _lib.check(_lib.lib().asuka_jk_class_stats(ctypes.c_void_p(self.handle),
                                                        rows.ctypes.data_as(_lib._i64p), cap))

@@ -35,6 +35,7 @@ def ncart(l: int) -> int:
 
 
 def flop_eri(la: int, lb: int, lc: int, ld: int, K: int) -> int:
+    la, lb, lc, ld, K = int(la), int(lb), int(lc), int(ld), int(K)
     L = la + lb + lc + ld
     n = L // 2 + 1
     ncomp = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
@@ -49,11 +50,13 @@ def _terms(bin_: int) -> tuple[int, int]:
 
 
 def flop_dig(la: int, lb: int, lc: int, ld: int, hist) -> int:
+    la, lb, lc, ld = int(la), int(lb), int(lc), int(ld)
     ncomp = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
     return sum(2 * ncomp * sum(_terms(b)) * int(c) for b, c in enumerate(hist))
 
 
 def bytes_alg(la: int, lb: int, lc: int, ld: int, hist) -> int:
+    la, lb, lc, ld = int(la), int(lb), int(lc), int(ld)
     nA, nB, nC, nD = ncart(la), ncart(lb), ncart(lc), ncart(ld)
     nAB, nCD = nA * nB, nC * nD
     total = 0
