@@ -177,8 +177,9 @@ def main(out_path: str) -> None:
     lines.append("// interval i for T < rys_tmax[n]; beyond it x_u = hx_u/T, w_u = hw_u/sqrt(T)")
     lines.append("// (2n-point Gauss-Hermite limit). Layout per n: [interval][2n functions][RYS_DEG+1],")
     lines.append("// functions 0..n-1 are roots (ascending), n..2n-1 the matching weights.")
-    lines.append("// Included once per address space: define RYS_TABLE_QUAL (e.g. `static const`")
-    lines.append("// or `static const __device__`) and wrap the include in a namespace.")
+    lines.append("// Included once per address space and coefficient type: define RYS_TABLE_QUAL")
+    lines.append("// (e.g. `static const` or `static const __device__`) and RYS_COEF_T (double, or")
+    lines.append("// float for an FP32 copy of the coefficients) and wrap the include in a namespace.")
     lines.append(f"#define RYS_NMAX {NMAX}")
     lines.append(f"#define RYS_DEG {DEG}")
     lines.append(f"#define RYS_H {H!r}")
@@ -205,7 +206,7 @@ def main(out_path: str) -> None:
     for n in range(1, NMAX + 1):
         flat.extend(tables[n].ravel().tolist())
     lines.append(f"#define RYS_COEF_TOTAL {len(flat)}")
-    lines.append(f"RYS_TABLE_QUAL double rys_coef[{len(flat)}] = {{")
+    lines.append(f"RYS_TABLE_QUAL RYS_COEF_T rys_coef[{len(flat)}] = {{")
     for i in range(0, len(flat), 4):
         lines.append("  " + ", ".join(repr(float(v)) for v in flat[i:i + 4]) + ",")
     lines.append("};")

@@ -28,6 +28,9 @@ hipError_t launch_eri_fast(const DevBasis& b, const ClassWork& w, const EriOut& 
 
 static thread_local std::string g_err;
 
+// default FP32 threshold on Q_ab Q_cd max|D| for builds with ASUKA_JK_MIXED
+static constexpr double kDefaultTauMixed = 1e-7;
+
 static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -202,7 +205,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
     ctx->shell_atom.assign(nshell, 0);
   ctx->eps_schwarz = opts->eps_schwarz;
   ctx->eps_density = opts->eps_density < 0.0 ? std::max(opts->eps_schwarz, 1e-10) : opts->eps_density;
-  ctx->tau_mixed = opts->tau_mixed;
+  ctx->tau_mixed = opts->tau_mixed > 0.0 ? opts->tau_mixed : kDefaultTauMixed;
 
   // ---- shell pairs, l-ordered orientation (asuka/cueri/shell_pairs.py:72-116) ----
   for (int i = 0; i < nshell; ++i)
@@ -544,6 +547,8 @@ int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int
   o.J = wantJ ? J : nullptr;
   o.K = wantK ? K : nullptr;
   o.eps_density = ctx->eps_density;
+  o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
+  o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
     const long long nc = kCountStride * (long long)std::max<size_t>(1, ctx->cps.size());
     hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(nc, 256)), dim3(256), 0, s, ctx->d_counters, nc);
@@ -573,6 +578,8 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   o.D = D;
   o.F = F;
   o.eps_density = ctx->eps_density;
+  o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
+  o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
     const long long nc = kCountStride * (long long)std::max<size_t>(1, ctx->cps.size());
     hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(nc, 256)), dim3(256), 0, s, ctx->d_counters, nc);

@@ -86,7 +86,10 @@ struct Range {
   static constexpr int index(int x, int y, int z) { return off(x + y + z) + cidx_(x, y, z); }
 };
 
-__device__ __forceinline__ double pw(const double* p, int k) { return p[k]; }
+template <typename CT>
+__device__ __forceinline__ CT pw(const CT* p, int k) {
+  return p[k];
+}
 
 template <int LA, int LB, int LC, int LD, int SA, int SC>
 struct Plan {
@@ -160,7 +163,27 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <class PL, int CH>
+// Arithmetic helpers for the compute type CT of the recursion (double, or
+// float on the FP32 path: native reciprocal / square root, FP32 root table).
+__device__ __forceinline__ double fmac(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double rcp_ct(double v) { return 1.0 / v; }
+__device__ __forceinline__ float rcp_ct(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double sqrt_ct(double v) { return sqrt(v); }
+__device__ __forceinline__ float sqrt_ct(float v) { return __builtin_amdgcn_sqrtf(v); }
+template <int N>
+__device__ __forceinline__ void roots_ct(double T, double* x, double* w) {
+  rys_roots_dev<N>(T, x, w);
+}
+template <int N>
+__device__ __forceinline__ void roots_ct(float T, float* x, float* w) {
+  rys_roots_dev_f<N>(T, x, w);
+}
+
+// One chunk of one shell quartet per lane. CT is the type of the primitive
+// setup, roots, VRR, contracted intermediates and HRR; the digestion always
+// accumulates in FP64. Geometric differences are formed in FP64 and rounded.
+template <class PL, int CH, typename CT>
 __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, bool active) {
   using CK = Chunk<PL, CH>;
   constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
@@ -173,6 +196,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   constexpr auto te = PL::RE::tab();
   constexpr auto tf = PL::RF::tab();
   constexpr auto L = CK::L;
+  const CT one = CT(1), half = CT(0.5);
 
   const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
   const double Ax = b.shell_xyz[3 * A], Ay = b.shell_xyz[3 * A + 1], Az = b.shell_xyz[3 * A + 2];
@@ -180,51 +204,52 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   const int ab0 = b.pp_start[P], nab = active ? b.pp_count[P] : 0;
   const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
 
-  double ef[NEL * NFL];
+  CT ef[NEL * NFL];
 #pragma unroll
-  for (int i = 0; i < NEL * NFL; ++i) ef[i] = 0.0;
+  for (int i = 0; i < NEL * NFL; ++i) ef[i] = CT(0);
 
   for (int i = 0; i < nab; ++i) {
     const PrimPair X = b.pp[ab0 + i];
-    const double PAx = X.Px - Ax, PAy = X.Py - Ay, PAz = X.Pz - Az;
+    const CT PAx = CT(X.Px - Ax), PAy = CT(X.Py - Ay), PAz = CT(X.Pz - Az);
+    const CT p = CT(X.eta), ip = CT(X.inv_eta), ckp = CT(X.cK * X.inv_eta);
     for (int j = 0; j < ncd; ++j) {
       const PrimPair Y = b.pp[cd0 + j];
-      const double p = X.eta, q = Y.eta;
-      const double inv_pq = 1.0 / (p + q);
-      const double dx = X.Px - Y.Px, dy = X.Py - Y.Py, dz = X.Pz - Y.Pz;
-      const double T = p * q * inv_pq * (dx * dx + dy * dy + dz * dz);
+      const CT q = CT(Y.eta), iq = CT(Y.inv_eta);
+      const CT inv_pq = rcp_ct(p + q);
+      const CT dx = CT(X.Px - Y.Px), dy = CT(X.Py - Y.Py), dz = CT(X.Pz - Y.Pz);
+      const CT T = p * q * inv_pq * (dx * dx + dy * dy + dz * dz);
       // 2 pi^(5/2) / (p q sqrt(p+q)) cK_ab cK_cd
-      const double base = 34.98683665524972497 * (X.cK * X.inv_eta) * (Y.cK * Y.inv_eta) * sqrt(inv_pq);
-      double rx[NR], rw[NR];
-      rys_roots_dev<NR>(T, rx, rw);
-      const double qf = q * inv_pq, pf = p * inv_pq;
-      const double QCx = Y.Px - Cx, QCy = Y.Py - Cy, QCz = Y.Pz - Cz;
+      const CT base = CT(34.98683665524972497) * ckp * CT(Y.cK * Y.inv_eta) * sqrt_ct(inv_pq);
+      CT rx[NR], rw[NR];
+      roots_ct<NR>(T, rx, rw);
+      const CT qf = q * inv_pq, pf = p * inv_pq;
+      const CT QCx = CT(Y.Px - Cx), QCy = CT(Y.Py - Cy), QCz = CT(Y.Pz - Cz);
 #pragma unroll
       for (int u = 0; u < NR; ++u) {
-        const double x = rx[u];
-        const double B0 = 0.5 * x * inv_pq;
-        const double B1 = 0.5 * (1.0 - x) * X.inv_eta + B0;
-        const double B1p = 0.5 * (1.0 - x) * Y.inv_eta + B0;
-        const double xq = qf * x, xp = pf * x;
-        double G[3][NMAX + 1][MMAX + 1];
-        const double c00[3] = {PAx - xq * dx, PAy - xq * dy, PAz - xq * dz};
-        const double c0p[3] = {QCx + xp * dx, QCy + xp * dy, QCz + xp * dz};
+        const CT x = rx[u];
+        const CT B0 = half * x * inv_pq;
+        const CT B1 = half * (one - x) * ip + B0;
+        const CT B1p = half * (one - x) * iq + B0;
+        const CT xq = qf * x, xp = pf * x;
+        CT G[3][NMAX + 1][MMAX + 1];
+        const CT c00[3] = {PAx - xq * dx, PAy - xq * dy, PAz - xq * dz};
+        const CT c0p[3] = {QCx + xp * dx, QCy + xp * dy, QCz + xp * dz};
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-          const double g0 = (ax == 2) ? rw[u] * base : 1.0;
-          const double Cc = c00[ax], Cp = c0p[ax];
+          const CT g0 = (ax == 2) ? rw[u] * base : one;
+          const CT Cc = c00[ax], Cp = c0p[ax];
           G[ax][0][0] = g0;
           if (NMAX > 0) G[ax][1][0] = Cc * g0;
 #pragma unroll
-          for (int a = 2; a <= NMAX; ++a) G[ax][a][0] = B1 * (a - 1) * G[ax][a - 2][0] + Cc * G[ax][a - 1][0];
+          for (int a = 2; a <= NMAX; ++a) G[ax][a][0] = B1 * CT(a - 1) * G[ax][a - 2][0] + Cc * G[ax][a - 1][0];
           if (MMAX > 0) {
 #pragma unroll
             for (int a = 0; a <= NMAX; ++a) {
-              G[ax][a][1] = Cp * G[ax][a][0] + (a > 0 ? a * B0 * G[ax][a > 0 ? a - 1 : 0][0] : 0.0);
+              G[ax][a][1] = Cp * G[ax][a][0] + (a > 0 ? CT(a) * B0 * G[ax][a > 0 ? a - 1 : 0][0] : CT(0));
 #pragma unroll
               for (int c = 2; c <= MMAX; ++c)
-                G[ax][a][c] = B1p * (c - 1) * G[ax][a][c - 2] + Cp * G[ax][a][c - 1] +
-                              (a > 0 ? a * B0 * G[ax][a > 0 ? a - 1 : 0][c - 1] : 0.0);
+                G[ax][a][c] = B1p * CT(c - 1) * G[ax][a][c - 2] + Cp * G[ax][a][c - 1] +
+                              (a > 0 ? CT(a) * B0 * G[ax][a > 0 ? a - 1 : 0][c - 1] : CT(0));
             }
           }
         }
@@ -233,8 +258,8 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
           for (int jf = 0; jf < NFL; ++jf) {
             const int eg = L.e[ie], fg = L.f[jf];
-            const double gxy = G[0][te.x[eg]][tf.x[fg]] * G[1][te.y[eg]][tf.y[fg]];
-            ef[ie * NFL + jf] = fma(gxy, G[2][te.z[eg]][tf.z[fg]], ef[ie * NFL + jf]);
+            const CT gxy = G[0][te.x[eg]][tf.x[fg]] * G[1][te.y[eg]][tf.y[fg]];
+            ef[ie * NFL + jf] = fmac(gxy, G[2][te.z[eg]][tf.z[fg]], ef[ie * NFL + jf]);
           }
         }
       }
@@ -244,21 +269,21 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   // ---- horizontal transfer: ket (C-components of the chunk) then bra ----
   const double Bx = b.shell_xyz[3 * B], By = b.shell_xyz[3 * B + 1], Bz = b.shell_xyz[3 * B + 2];
   const double Dx = b.shell_xyz[3 * Dsh], Dy = b.shell_xyz[3 * Dsh + 1], Dz = b.shell_xyz[3 * Dsh + 2];
-  double pab[3][LB + 1], pcd[3][LD + 1];
+  CT pab[3][LB + 1], pcd[3][LD + 1];
   {
-    const double ab[3] = {Ax - Bx, Ay - By, Az - Bz};
-    const double cd[3] = {Cx - Dx, Cy - Dy, Cz - Dz};
+    const CT ab[3] = {CT(Ax - Bx), CT(Ay - By), CT(Az - Bz)};
+    const CT cd[3] = {CT(Cx - Dx), CT(Cy - Dy), CT(Cz - Dz)};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-      pab[ax][0] = 1.0;
-      pcd[ax][0] = 1.0;
+      pab[ax][0] = one;
+      pcd[ax][0] = one;
 #pragma unroll
       for (int e = 1; e <= LB; ++e) pab[ax][e] = pab[ax][e - 1] * ab[ax];
 #pragma unroll
       for (int e = 1; e <= LD; ++e) pcd[ax][e] = pcd[ax][e - 1] * cd[ax];
     }
   }
-  double ecd[NEL * NCDL];
+  CT ecd[NEL * NCDL];
 #pragma unroll
   for (int ie = 0; ie < NEL; ++ie)
 #pragma unroll
@@ -267,20 +292,20 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
       for (int id = 0; id < ND; ++id) {
         const int cxv = cx_(LC, c0 + ic), cyv = cy_(LC, c0 + ic), czv = LC - cxv - cyv;
         const int dxv = cx_(LD, id), dyv = cy_(LD, id), dzv = LD - dxv - dyv;
-        double s = 0.0;
+        CT s = CT(0);
 #pragma unroll
         for (int mx = 0; mx <= dxv; ++mx)
 #pragma unroll
           for (int my = 0; my <= dyv; ++my)
 #pragma unroll
             for (int mz = 0; mz <= dzv; ++mz) {
-              const double coef = (double)(binom_(dxv, mx) * binom_(dyv, my) * binom_(dzv, mz)) *
+              const CT coef = CT(binom_(dxv, mx) * binom_(dyv, my) * binom_(dzv, mz)) *
                                   pw(pcd[0], dxv - mx) * pw(pcd[1], dyv - my) * pw(pcd[2], dzv - mz);
-              s = fma(coef, ef[ie * NFL + L.fmap[PL::RF::index(cxv + mx, cyv + my, czv + mz)]], s);
+              s = fmac(coef, ef[ie * NFL + L.fmap[PL::RF::index(cxv + mx, cyv + my, czv + mz)]], s);
             }
         ecd[ie * NCDL + ic * ND + id] = s;
       }
-  double tile[NAL * NB * NCDL];
+  CT tile[NAL * NB * NCDL];
 #pragma unroll
   for (int ia = 0; ia < NAL; ++ia)
 #pragma unroll
@@ -289,16 +314,16 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
       const int bxv = cx_(LB, ib), byv = cy_(LB, ib), bzv = LB - bxv - byv;
 #pragma unroll
       for (int jc = 0; jc < NCDL; ++jc) {
-        double s = 0.0;
+        CT s = CT(0);
 #pragma unroll
         for (int nx = 0; nx <= bxv; ++nx)
 #pragma unroll
           for (int ny = 0; ny <= byv; ++ny)
 #pragma unroll
             for (int nz = 0; nz <= bzv; ++nz) {
-              const double coef = (double)(binom_(bxv, nx) * binom_(byv, ny) * binom_(bzv, nz)) *
+              const CT coef = CT(binom_(bxv, nx) * binom_(byv, ny) * binom_(bzv, nz)) *
                                   pw(pab[0], bxv - nx) * pw(pab[1], byv - ny) * pw(pab[2], bzv - nz);
-              s = fma(coef, ecd[L.emap[PL::RE::index(axv + nx, ayv + ny, azv + nz)] * NCDL + jc], s);
+              s = fmac(coef, ecd[L.emap[PL::RE::index(axv + nx, ayv + ny, azv + nz)] * NCDL + jc], s);
             }
         tile[(ia * NB + ib) * NCDL + jc] = s;
       }
@@ -334,7 +359,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
     for (int r = 0; r < NAL * NB; ++r) {
       double s = 0.0;
 #pragma unroll
-      for (int c = 0; c < NCDL; ++c) s = fma(tile[r * NCDL + c], dcd[c], s);
+      for (int c = 0; c < NCDL; ++c) s = fma((double)tile[r * NCDL + c], dcd[c], s);
       s = active ? fj * s : 0.0;
       if (uniform) {
         s = wave_sum(s);
@@ -353,7 +378,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
       for (int c = 0; c < NCDL; ++c) {
         double s = 0.0;
 #pragma unroll
-        for (int r = 0; r < NAL * NB; ++r) s = fma(tile[r * NCDL + c], dab[r], s);
+        for (int r = 0; r < NAL * NB; ++r) s = fma((double)tile[r * NCDL + c], dab[r], s);
         atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
       }
     }
@@ -373,7 +398,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
           for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-            for (int id = 0; id < ND; ++id) s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dbd[ib * ND + id], s);
+            for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbd[ib * ND + id], s);
           atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], ks * s);
         }
     }
@@ -391,7 +416,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
           for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-            for (int ic = 0; ic < NCL; ++ic) s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dbc[ib * NCL + ic], s);
+            for (int ic = 0; ic < NCL; ++ic) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbc[ib * NCL + ic], s);
           atomicAdd(&Ko[(aoA + ia) * N + aoD + id], ks * s);
         }
     }
@@ -409,7 +434,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
           for (int ia = 0; ia < NAL; ++ia)
 #pragma unroll
-            for (int id = 0; id < ND; ++id) s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dad[ia * ND + id], s);
+            for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dad[ia * ND + id], s);
           atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], ks * s);
         }
       if (cd_neq) {  // K[b,d] += sum_{a,c} v D[a,c]
@@ -427,7 +452,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
             for (int ia = 0; ia < NAL; ++ia)
 #pragma unroll
               for (int ic = 0; ic < NCL; ++ic)
-                s = fma(tile[(ia * NB + ib) * NCDL + ic * ND + id], dac[ia * NCL + ic], s);
+                s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dac[ia * NCL + ic], s);
             atomicAdd(&Ko[(aoB + ib) * N + aoD + id], ks * s);
           }
       }
@@ -435,24 +460,32 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   }
 }
 
-template <class PL, int CH>
+template <class PL, int CH, typename CT>
 __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp,
                                                bool active) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH>(b, o, P, Qp, active);
+      run_chunk<PL, CH, CT>(b, o, P, Qp, active);
       return;
     }
-    dispatch_chunk<PL, CH + 1>(chunk, b, o, P, Qp, active);
+    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, active);
   }
 }
 
 // Work mapping: wavefront g of the grid-stride loop takes 64 consecutive
 // quartets (one per lane) and one chunk, chunk = g mod NCH, so the chunk (and
 // with it all register indexing) is uniform across the wave.
-template <class PL>
+//
+// Precision: with o.mixed a wave votes FP32 when every active quartet's bound
+// Q_ab Q_cd max|D| is below tau. The FP64 instantiation (CT = double) takes
+// the waves that vote FP64 (all waves when o.mixed is off), the FP32
+// instantiation the others; the vote is a pure function of the wave's
+// quartets, so every wave is processed by exactly one of the two launches,
+// each with its own register allocation.
+template <class PL, typename CT>
 __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o) {
   constexpr int NCH = PL::NCH;
+  constexpr bool kFP32 = sizeof(CT) == 4;
   const long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long wave = gtid >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
@@ -464,22 +497,29 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, 
     bool active = k < w.nwork;
     int P = 0, Qp = 0;
     if (active) active = decode_work(w, k, P, Qp);
+    double bound = 0.0;
     if (active) {
-      if (o.eps_density > 0.0) {
-        const double bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
-        active = bound >= o.eps_density;
-      }
+      bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
+      if (o.eps_density > 0.0) active = bound >= o.eps_density;
     }
+    const bool lowp = o.mixed && __all(!active || bound < o.tau_mixed);
+    if (lowp != kFP32) continue;  // the other instantiation owns this wave
     if (o.counters && chunk == 0 && active) {
       const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
       unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
+      const unsigned long long kq = (unsigned long long)b.pp_count[P] * (unsigned long long)b.pp_count[Qp];
       atomicAdd(cnt + (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp)), 1ull);
-      atomicAdd(cnt + 8, (unsigned long long)b.pp_count[P] * (unsigned long long)b.pp_count[Qp]);
+      atomicAdd(cnt + 8, kq);
+      if (kFP32) {
+        atomicAdd(cnt + 9, 1ull);
+        atomicAdd(cnt + 10, kq);
+      }
     }
     // inactive lanes (tail, screened) run with empty primitive loops and no
     // atomics so the wave stays converged for the cross-lane reductions
-    dispatch_chunk<PL, 0>(chunk, b, o, P, Qp, active);
+    dispatch_chunk<PL, 0, CT>(chunk, b, o, P, Qp, active);
   }
+  (void)lane;
 }
 
 template <class PL>
@@ -489,7 +529,8 @@ hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, h
   long long g = (waves + 3) / 4;
   if (g > 16384) g = 16384;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(block), 0, s, b, w, o);
+  hipLaunchKernelGGL((eri_fast_kernel<PL, double>), dim3((unsigned)g), dim3(block), 0, s, b, w, o);
+  if (o.mixed) hipLaunchKernelGGL((eri_fast_kernel<PL, float>), dim3((unsigned)g), dim3(block), 0, s, b, w, o);
   return hipGetLastError();
 }
 

@@ -88,8 +88,10 @@ struct EriOut {
   double eps_density;  // <= 0 disables density screening
   // optional statistics, kCountStride per class pair: [0..7] quartets evaluated
   // by symmetry bin (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair), [8] their
-  // primitive quartets
+  // primitive quartets, [9] quartets evaluated in FP32, [10] their primitive quartets
   unsigned long long* counters;
+  int mixed;          // allow the FP32 path
+  double tau_mixed;   // FP32 when Q_ab Q_cd max|D| < tau_mixed for the whole wave
 };
 
 constexpr int kCountStride = 16;

@@ -11,13 +11,26 @@
 
 namespace rys_dev {
 #define RYS_TABLE_QUAL static const __device__
+#define RYS_COEF_T double
 #include "rys_table.h"
+#undef RYS_COEF_T
 #undef RYS_TABLE_QUAL
 }  // namespace rys_dev
 
+// FP32 copy of the polynomial coefficients (same layout) for the FP32 path
+namespace rys_dev_f {
+#define RYS_TABLE_QUAL static const __device__
+#define RYS_COEF_T float
+#include "rys_table.h"
+#undef RYS_COEF_T
+#undef RYS_TABLE_QUAL
+}  // namespace rys_dev_f
+
 namespace rys_host {
 #define RYS_TABLE_QUAL static const
+#define RYS_COEF_T double
 #include "rys_table.h"
+#undef RYS_COEF_T
 #undef RYS_TABLE_QUAL
 }  // namespace rys_host
 
@@ -76,7 +89,38 @@ __device__ inline void rys_roots_dev(double T, double* x, double* w) {
       x[f] = a;
     else
       w[f - N] = a;
-    // keep one function's coefficients live at a time (bounds VGPR use)
-    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// All n roots in FP32 (the FP32 path of mixed precision). Below the
+// asymptotic switch the FP32 coefficient copy is evaluated in FP32; at large T
+// (where the Gauss-Hermite limit applies) x and w are formed in FP64 and
+// rounded, so the large-argument roots keep FP64 accuracy before rounding.
+template <int N>
+__device__ inline void rys_roots_dev_f(float T, float* x, float* w) {
+  if (T >= (float)rys_dev::rys_tmax[N]) {
+    const double it = 1.0 / (double)T;
+    const double sit = sqrt(it);
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      x[u] = (float)(rys_dev::rys_herm_x[rys_dev::rys_herm_off[N] + u] * it);
+      w[u] = (float)(rys_dev::rys_herm_w[rys_dev::rys_herm_off[N] + u] * sit);
+    }
+    return;
+  }
+  int i = (int)(T * (float)RYS_INV_H);
+  if (i >= rys_dev::rys_nint[N]) i = rys_dev::rys_nint[N] - 1;
+  const float s = (T - (float)i * (float)RYS_H) * (float)(2.0 * RYS_INV_H) - 1.0f;
+  const float* c = rys_dev_f::rys_coef + rys_dev::rys_coef_off[N] + i * 2 * N * (RYS_DEG + 1);
+#pragma unroll
+  for (int f = 0; f < 2 * N; ++f) {
+    const float* cf = c + f * (RYS_DEG + 1);
+    float a = cf[RYS_DEG];
+#pragma unroll
+    for (int d = RYS_DEG - 1; d >= 0; --d) a = __builtin_fmaf(a, s, cf[d]);
+    if (f < N)
+      x[f] = a;
+    else
+      w[f - N] = a;
   }
 }

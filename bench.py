@@ -82,7 +82,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--mode", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -113,48 +113,72 @@ def main() -> int:
     log(f"context: nao={nao} nsp={ctx.nsp} schwarz-screened quartets={ctx.ntasks} ({t_ctx:.2f}s)")
 
     D = torch.as_tensor(D_h, device=dev)
-    J = torch.empty((nao, nao), dtype=torch.float64, device=dev)
-    K = torch.empty_like(J)
     L = _lib.lib()
     h = ctypes.c_void_p(ctx.handle)
     stream = torch.cuda.current_stream(dev)
     sptr = ctypes.c_void_p(stream.cuda_stream)
-    base_flags = _lib.WANT_J | _lib.WANT_K | (_lib.MIXED if args.mode == "mixed" else 0)
 
-    def build(extra=0):
-        _lib.check(L.asuka_jk_build(h, _lib.ptr(D), _lib.ptr(J), _lib.ptr(K), base_flags | extra, sptr))
+    def run_mode(mode: str):
+        J = torch.empty((nao, nao), dtype=torch.float64, device=dev)
+        K = torch.empty_like(J)
+        flags = _lib.WANT_J | _lib.WANT_K | (_lib.MIXED if mode == "mixed" else 0)
 
-    for _ in range(max(0, args.warmup)):
-        build()
-    torch.cuda.synchronize(dev)
-    # counted build: exact per-class work of this density
-    build(_lib.COUNT)
-    stats = ctx.class_stats()
-    ncp = len(stats)
-    rows = np.zeros((max(ncp, 1), 16), dtype=np.int64)
-    L.asuka_jk_last_counts(h, rows.ctypes.data_as(_lib._i64p), ncp, sptr)
-    class_rows = [(s["class"], rows[i, :8].tolist(), int(rows[i, 8])) for i, s in enumerate(stats)]
-    work = metrics.build_work(class_rows)
-    log(f"work: quartets={work.quartets} prim={work.prim_quartets} EFM-v1 GFLOP={work.flops / 1e9:.2f}")
+        def build(extra=0):
+            _lib.check(L.asuka_jk_build(h, _lib.ptr(D), _lib.ptr(J), _lib.ptr(K), flags | extra, sptr))
 
-    # timed region
-    torch.cuda.synchronize(dev)
-    kern_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        build(_lib.TIME)
-        ms = ctypes.c_double()
-        nl = ctypes.c_int64()
-        _lib.check(L.asuka_jk_last_timing(h, ctypes.byref(ms), ctypes.byref(nl)))
-        kern_ms.append(ms.value)
-    torch.cuda.synchronize(dev)
-    wall = time.perf_counter() - t0
-    s_per_build = wall / args.steps
-    kern_s = float(np.mean(kern_ms)) / 1e3
+        for _ in range(max(0, args.warmup)):
+            build()
+        torch.cuda.synchronize(dev)
+        build(_lib.COUNT)  # counted build: exact per-class work of this density
+        stats = ctx.class_stats()
+        ncp = len(stats)
+        rows = np.zeros((max(ncp, 1), 16), dtype=np.int64)
+        L.asuka_jk_last_counts(h, rows.ctypes.data_as(_lib._i64p), ncp, sptr)
+        class_rows = [(s_["class"], rows[i, :8].tolist(), int(rows[i, 8])) for i, s_ in enumerate(stats)]
+        work = metrics.build_work(class_rows)
+        f32 = sum(metrics.flop_eri(*s_["class"], int(rows[i, 10])) for i, s_ in enumerate(stats))
+        q32 = int(rows[:, 9].sum())
+        torch.cuda.synchronize(dev)
+        kern_ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            build(_lib.TIME)
+            ms = ctypes.c_double()
+            nl = ctypes.c_int64()
+            _lib.check(L.asuka_jk_last_timing(h, ctypes.byref(ms), ctypes.byref(nl)))
+            kern_ms.append(ms.value)
+        torch.cuda.synchronize(dev)
+        wall = time.perf_counter() - t0
+        r = {"J": J, "K": K, "work": work, "f32": f32, "q32": q32, "s_per_build": wall / args.steps,
+             "kern_s": float(np.mean(kern_ms)) / 1e3, "nlaunch": int(nl.value)}
+        log(f"{mode}: {r['s_per_build'] * 1e3:.2f} ms/build, ERI kernels {r['kern_s'] * 1e3:.2f} ms over "
+            f"{r['nlaunch']} launches, quartets={work.quartets} (FP32 {q32}), EFM-v1 GFLOP={work.flops / 1e9:.2f}")
+        return r
+
+    modes = ["fp64", "mixed"] if args.mode == "both" else [args.mode]
+    res = {m: run_mode(m) for m in modes}
+    main_mode = "mixed" if "mixed" in res else "fp64"
+    r = res[main_mode]
+    work = r["work"]
+    s_per_build = r["s_per_build"]
+    kern_s = r["kern_s"]
     value = work.quartets / s_per_build
-    f64 = work.flops if args.mode == "fp64" else work.flops  # mixed split is recorded per quartet once implemented
+    f32 = r["f32"]
+    f64 = work.flops - f32
     achieved = work.flops / kern_s / 1e12
-    log(f"timed: {s_per_build * 1e3:.2f} ms/build, ERI kernels {kern_s * 1e3:.2f} ms over {nl.value} launches")
+    accuracy = None
+    if "fp64" in res and "mixed" in res:
+        dJ = res["mixed"]["J"] - res["fp64"]["J"]
+        dK = res["mixed"]["K"] - res["fp64"]["K"]
+        dE = float(0.5 * (D * dJ).sum() - 0.25 * (D * dK).sum())
+        accuracy = {"max_abs_dJ": float(dJ.abs().max()), "max_abs_dK": float(dK.abs().max()),
+                    "dE_2e_Eh": dE, "tau_mixed": 1e-7,
+                    "fp64_s_per_build": res["fp64"]["s_per_build"],
+                    "mixed_s_per_build": res["mixed"]["s_per_build"],
+                    "speedup_mixed_vs_fp64": res["fp64"]["s_per_build"] / res["mixed"]["s_per_build"],
+                    "fp32_quartet_fraction": r["q32"] / max(1, work.quartets)}
+        log("mixed vs fp64:", json.dumps(accuracy))
+    nl = type("N", (), {"value": r["nlaunch"]})()
 
     # parity stamp: sampled tiles vs the oracle
     parity = {}
@@ -202,18 +226,20 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.mode == "fp64" else "f64+f32",
+        "dtype": "f64" if main_mode == "fp64" else "f64+f32",
         "data": "synthetic (seeded water cluster geometry, random orthonormal occupied orbitals)",
         "config": {"workload": cfg["workload"], "nao": nao, "basis": cfg["basis"], "n_water": cfg["n"],
                    "geometry_seed": cfg["seed"], "density_seed": cfg["cseed"], "eps_schwarz": cfg["eps"],
-                   "eps_density": ctx.eps_density, "mode": args.mode},
+                   "eps_density": ctx.eps_density, "mode": main_mode},
         "s_per_build": s_per_build,
         "quartets_per_build": work.quartets,
         "schwarz_screened_quartets": ctx.ntasks,
         "prim_quartet_roots_per_s": work.prim_roots / s_per_build,
         "context_build_s": t_ctx,
+        "mixed_vs_fp64": accuracy,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": metrics.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": metrics.roofline_fraction(f64, 0.0, kern_s), "traffic": None,
+                     "frac": metrics.roofline_fraction(f64, f32, kern_s), "traffic": None,
+                     "flops_fp64": f64, "flops_fp32": f32,
                      "flops_per_build_efm_v1": work.flops, "kernel_ms_per_build": kern_s * 1e3,
                      "kernel_launches_per_build": int(nl.value),
                      "alg_bytes_per_build": work.bytes_alg},

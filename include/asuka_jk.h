@@ -74,8 +74,10 @@ typedef struct {
   double eps_schwarz; /* keep quartets with Q_ab*Q_cd >= eps; 0 => Q := 1, no screening */
   double eps_density; /* keep quartets with Q_ab*Q_cd*max(Dmax_ab, Dmax_cd) >= eps_density;
                          < 0 => max(eps_schwarz, 1e-10) (asuka/hf/direct_jk.py:1594-1596); 0 disables */
-  double tau_mixed;   /* with ASUKA_JK_MIXED: quartets whose Q_ab*Q_cd*max|D| bound is below
-                         tau_mixed run the Rys recursion in FP32 */
+  double tau_mixed;   /* with ASUKA_JK_MIXED: a wavefront of quartets whose Q_ab*Q_cd*max|D|
+                         bounds are all below tau_mixed runs the primitive setup, Rys roots,
+                         recursion and transfer in FP32 (J/K accumulation stays FP64);
+                         <= 0 => 1e-7 */
   int32_t reserved[8];
 } asuka_jk_opts;
 
@@ -118,7 +120,8 @@ int asuka_jk_class_stats(const asuka_jk_ctx* ctx, int64_t* rows, int cap);
  * class pair in asuka_jk_class_stats order: [0..7] quartets evaluated after
  * density screening, binned by (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair)
  * (the symmetry histogram of asuka/hf/direct_jk_cost.py:134-294), [8] their
- * primitive quartets, [9..15] 0. Synchronises the stream. Returns the number of rows. */
+ * primitive quartets, [9] quartets evaluated in FP32 (ASUKA_JK_MIXED), [10] their
+ * primitive quartets, [11..15] 0. Synchronises the stream. Returns the number of rows. */
 int asuka_jk_last_counts(asuka_jk_ctx* ctx, int64_t* rows, int cap, void* stream);
 
 /* Milliseconds between the HIP events recorded (on the build's stream) before
