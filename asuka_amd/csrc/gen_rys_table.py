@@ -188,6 +188,10 @@ def main(out_path: str) -> None:
     offs = [0]
     for n in range(1, NMAX + 1):
         offs.append(offs[-1] + nints[n] * 2 * n * (DEG + 1))
+    # the same metadata as macro lists, for compile-time use (constexpr copies)
+    lines.append("#define RYS_NINT_LIST " + ", ".join(str(v) for v in nints))
+    lines.append("#define RYS_TMAX_LIST 0.0, " + ", ".join(f"{tmax[n]!r}" for n in range(1, NMAX + 1)))
+    lines.append("#define RYS_COEF_OFF_LIST 0, " + ", ".join(str(v) for v in offs[:-1]))
     lines.append("RYS_TABLE_QUAL int rys_nint[RYS_NMAX + 1] = {" + ", ".join(str(v) for v in nints) + "};")
     lines.append("RYS_TABLE_QUAL double rys_tmax[RYS_NMAX + 1] = {0.0, " +
                  ", ".join(f"{tmax[n]!r}" for n in range(1, NMAX + 1)) + "};")

@@ -30,6 +30,8 @@ static thread_local std::string g_err;
 
 // default FP32 threshold on Q_ab Q_cd max|D| for builds with ASUKA_JK_MIXED
 static constexpr double kDefaultTauMixed = 1e-7;
+// sqrt(2 pi^(5/2)), split evenly over the bra and ket primitive-pair factors
+static constexpr double kSqrt2Pi52 = 5.91496717279561287782;
 
 static int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -51,6 +53,7 @@ struct ClassPairHost {
   int la, lb, lc, ld;
   int seg_start, nseg;
   long long nwork;      // padded work index range
+  long long unit_start, nunits;  // bra-uniform wavefront units
   long long nquartets;  // Schwarz-screened quartets
   long long nprim_quartets;
   long long nprim_roots;
@@ -124,6 +127,12 @@ int dev_upload(Tp** dst, const std::vector<Tp>& src, std::vector<void*>& owned, 
 }  // namespace
 
 hipError_t launch_int1e(const asuka_jk_ctx* ctx, double* S, double* T, double* V, hipStream_t s);
+hipError_t launch_eri_deriv(const DevBasis& b, const ClassWork& w, const double* D, const double* WJ,
+                            const double* WK, const int* shell_atom, double* grad, hipStream_t s);
+hipError_t launch_int1e_grad(const DevBasis& b, const int* pstart, const int* pnum, const double* pexp,
+                             const double* pcoef, int natm, const double* axyz, const double* aZ,
+                             const int* shell_atom, const double* Dh, const double* Ws, double* grad,
+                             hipStream_t s);
 
 struct asuka_jk_ctx {
   int device = 0;
@@ -132,6 +141,7 @@ struct asuka_jk_ctx {
   std::vector<int> shell_l, shell_ao, shell_atom, sp_A, sp_B, pp_start, pp_count;
   std::vector<double> shell_xyz, Q, atom_xyz, atom_Z;
   std::vector<PrimPair> pp;
+  std::vector<PrimPairF> ppf;
   std::vector<ClassPairHost> cps;
   long long ntasks = 0;
   DevBasis dev{};
@@ -139,6 +149,7 @@ struct asuka_jk_ctx {
   long long* d_row_off = nullptr;
   int* d_col_pair = nullptr;
   Segment* d_segs = nullptr;
+  WaveUnit* d_units = nullptr;
   double* d_Q = nullptr;
   double* d_Dsp = nullptr;
   double* d_atom_xyz = nullptr;
@@ -239,7 +250,20 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
         r.a_exp = al;
         r.b_exp = be;
         r.inv_eta = ie;
-        if (r.cK != 0.0) ctx->pp.push_back(r);
+        if (r.cK != 0.0) {
+          ctx->pp.push_back(r);
+          PrimPairF f{};
+          f.eta = eta;
+          f.inv_eta = ie;
+          f.ck = kSqrt2Pi52 * r.cK * ie;
+          f.Px = r.Px;
+          f.Py = r.Py;
+          f.Pz = r.Pz;
+          f.PAx = r.Px - a3[0];
+          f.PAy = r.Py - a3[1];
+          f.PAz = r.Pz - a3[2];
+          ctx->ppf.push_back(f);
+        }
       }
     ctx->pp_count.push_back((int)ctx->pp.size() - ctx->pp_start.back());
   }
@@ -250,6 +274,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   double* d_xyz;
   int *d_l, *d_ao, *d_spA, *d_spB, *d_pps, *d_ppc;
   PrimPair* d_pp;
+  PrimPairF* d_ppf;
   if ((rc = dev_upload(&d_xyz, ctx->shell_xyz, own, s))) return rc;
   if ((rc = dev_upload(&d_l, ctx->shell_l, own, s))) return rc;
   if ((rc = dev_upload(&d_ao, ctx->shell_ao, own, s))) return rc;
@@ -258,6 +283,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   if ((rc = dev_upload(&d_pps, ctx->pp_start, own, s))) return rc;
   if ((rc = dev_upload(&d_ppc, ctx->pp_count, own, s))) return rc;
   if ((rc = dev_upload(&d_pp, ctx->pp, own, s))) return rc;
+  if ((rc = dev_upload(&d_ppf, ctx->ppf, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_shell_atom, ctx->shell_atom, own, s))) return rc;
   {
     std::vector<int> ps(bd->shell_prim_start, bd->shell_prim_start + nshell), pn(bd->shell_nprim, bd->shell_nprim + nshell);
@@ -283,6 +309,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   dv.pp_start = d_pps;
   dv.pp_count = d_ppc;
   dv.pp = d_pp;
+  dv.ppf = d_ppf;
   dv.Q = ctx->d_Q;
   dv.Dsp = ctx->d_Dsp;
   dv.nao = ctx->nao;
@@ -341,6 +368,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   std::vector<int> col_all, row_pair_all;
   std::vector<long long> row_off_all;
   std::vector<Segment> segs;
+  std::vector<WaveUnit> units;
   for (int c = 0; c < ncls; ++c) {
     std::vector<int> kvals;
     for (int sp : cls_pairs[c]) kvals.push_back(ctx->pp_count[sp]);
@@ -370,6 +398,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       cp.lc = cls_la[Y];
       cp.ld = cls_lb[Y];
       cp.seg_start = (int)segs.size();
+      cp.unit_start = (long long)units.size();
       const int nroots = (cp.la + cp.lb + cp.lc + cp.ld) / 2 + 1;
       long long padded = 0;
       for (const PairGroup& bg : groups[X]) {
@@ -395,6 +424,8 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
             rowp.push_back(P);
             offs.push_back(off);
             off += len;
+            for (long long c0 = 0; c0 < len; c0 += 64)
+              units.push_back(WaveUnit{P, kg.col_start + (int)c0, (int)std::min<long long>(64, len - c0), 0});
           }
           if (off == 0) continue;
           Segment sg{};
@@ -416,6 +447,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
         }
       }
       cp.nseg = (int)segs.size() - cp.seg_start;
+      cp.nunits = (long long)units.size() - cp.unit_start;
       if (cp.nseg == 0) continue;
       cp.nwork = padded;
       cp.nprim_roots = cp.nprim_quartets * nroots;
@@ -424,6 +456,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
     }
   }
   if ((rc = dev_upload(&ctx->d_segs, segs, own, s))) return rc;
+  if ((rc = dev_upload(&ctx->d_units, units, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_pair, row_pair_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_off, row_off_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_col_pair, col_all, own, s))) return rc;
@@ -500,6 +533,8 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   w.row_off = ctx->d_row_off;
   w.col_pair = ctx->d_col_pair;
   w.nwork = cp.nwork;
+  w.units = ctx->d_units + cp.unit_start;
+  w.nunits = cp.nunits;
   return w;
 }
 
@@ -640,6 +675,33 @@ int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream
   if (!ctx) return fail(ASUKA_E_INVALID, "ctx must be non-null");
   if (V && ctx->natm <= 0) return fail(ASUKA_E_INVALID, "V needs atoms (natm > 0)");
   HIP_CHECK(launch_int1e(ctx, S, T, V, (hipStream_t)stream));
+  return ASUKA_OK;
+}
+
+int asuka_jk_grad(asuka_jk_ctx* ctx, const double* D, const double* WJ, const double* WK, double* grad,
+                  void* stream) {
+  if (!ctx || !D || !grad) return fail(ASUKA_E_INVALID, "ctx, D and grad must be non-null");
+  if (ctx->natm <= 0) return fail(ASUKA_E_INVALID, "the gradient needs atoms (natm > 0)");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, grad, (long long)3 * ctx->natm, 0.0);
+  HIP_CHECK(hipGetLastError());
+  if (!WJ && !WK) return ASUKA_OK;
+  for (size_t i = 0; i < ctx->cps.size(); ++i) {
+    ClassWork w = work_of(ctx, ctx->cps[i], (int)i);
+    HIP_CHECK(launch_eri_deriv(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, grad, s));
+  }
+  return ASUKA_OK;
+}
+
+int asuka_int1e_grad(asuka_jk_ctx* ctx, const double* Dh, const double* Ws, double* grad, void* stream) {
+  if (!ctx || !grad) return fail(ASUKA_E_INVALID, "ctx and grad must be non-null");
+  if (ctx->natm <= 0) return fail(ASUKA_E_INVALID, "the gradient needs atoms (natm > 0)");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, grad, (long long)3 * ctx->natm, 0.0);
+  HIP_CHECK(hipGetLastError());
+  if (!Dh && !Ws) return ASUKA_OK;
+  HIP_CHECK(launch_int1e_grad(ctx->dev, ctx->d_prim_start, ctx->d_prim_num, ctx->d_prim_exp, ctx->d_prim_coef,
+                              ctx->natm, ctx->d_atom_xyz, ctx->d_atom_Z, ctx->d_shell_atom, Dh, Ws, grad, s));
   return ASUKA_OK;
 }
 

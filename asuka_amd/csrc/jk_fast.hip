@@ -12,13 +12,13 @@ asuka_fast::Launcher find_launcher_p6(int la, int lb, int lc, int ld);
 
 hipError_t launch_eri_fast(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s, bool* handled) {
   *handled = false;
-  if (w.q_ab || (o.mode != MODE_JK && o.mode != MODE_FOCK)) return hipSuccess;
+  if (w.q_ab || !w.units || (o.mode != MODE_JK && o.mode != MODE_FOCK)) return hipSuccess;
   asuka_fast::Launcher f = nullptr;
   asuka_fast::Launcher (*finders[])(int, int, int, int) = {find_launcher_p0, find_launcher_p1, find_launcher_p2, find_launcher_p3, find_launcher_p4, find_launcher_p5, find_launcher_p6};
   for (auto fn : finders)
     if (!f) f = fn(w.la, w.lb, w.lc, w.ld);
   if (!f) return hipSuccess;
   *handled = true;
-  if (w.nwork <= 0) return hipSuccess;
+  if (w.nunits <= 0) return hipSuccess;
   return f(b, w, o, s);
 }

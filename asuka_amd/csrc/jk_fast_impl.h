@@ -156,6 +156,14 @@ struct Chunk {
   static constexpr int NEL = L.ne, NFL = L.nf;
 };
 
+// Wave votes through the ballot intrinsic, whose result the compiler knows to
+// be wave-uniform (branches on it stay scalar).
+__device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0ull; }
+__device__ __forceinline__ bool wave_all(bool v) { return __builtin_amdgcn_ballot_w64(!v) == 0ull; }
+
+// A value known to be the same on every lane, moved to a scalar register.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Sum over the 64 lanes of the wavefront (all lanes must be converged).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -164,27 +172,23 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Arithmetic helpers for the compute type CT of the recursion (double, or
-// float on the FP32 path: native reciprocal / square root, FP32 root table).
+// float on the FP32 path).
 __device__ __forceinline__ double fmac(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double rcp_ct(double v) { return 1.0 / v; }
-__device__ __forceinline__ float rcp_ct(float v) { return __builtin_amdgcn_rcpf(v); }
-__device__ __forceinline__ double sqrt_ct(double v) { return sqrt(v); }
-__device__ __forceinline__ float sqrt_ct(float v) { return __builtin_amdgcn_sqrtf(v); }
-template <int N>
-__device__ __forceinline__ void roots_ct(double T, double* x, double* w) {
-  rys_roots_dev<N>(T, x, w);
-}
-template <int N>
-__device__ __forceinline__ void roots_ct(float T, float* x, float* w) {
-  rys_roots_dev_f<N>(T, x, w);
-}
 
-// One chunk of one shell quartet per lane. CT is the type of the primitive
+// Root tables of up to this many roots are copied to LDS by each workgroup.
+constexpr int kLdsRootsMax = 3;
+
+// One chunk of one shell quartet per lane; the bra pair P is the same on every
+// lane of the wave (bra-uniform wave units), so the bra primitive pairs are
+// read with scalar loads in the inner loop while each lane streams its own
+// ket primitive pairs in the outer loop. CT is the type of the primitive
 // setup, roots, VRR, contracted intermediates and HRR; the digestion always
 // accumulates in FP64. Geometric differences are formed in FP64 and rounded.
+// `tab` is the NR-root block of the root table in CT (LDS or global).
 template <class PL, int CH, typename CT>
-__device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, bool active) {
+__device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, bool active,
+                                          const CT* __restrict__ tab) {
   using CK = Chunk<PL, CH>;
   constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
   constexpr int NMAX = PL::NMAX, MMAX = PL::MMAX;
@@ -198,32 +202,39 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   constexpr auto L = CK::L;
   const CT one = CT(1), half = CT(0.5);
 
-  const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
+  // bra-side indices are wave-uniform; readfirstlane states it so that the
+  // loads addressed by them are scalar
+  const int A = uni(b.sp_A[P]), B = uni(b.sp_B[P]), C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
   const double Ax = b.shell_xyz[3 * A], Ay = b.shell_xyz[3 * A + 1], Az = b.shell_xyz[3 * A + 2];
   const double Cx = b.shell_xyz[3 * C], Cy = b.shell_xyz[3 * C + 1], Cz = b.shell_xyz[3 * C + 2];
-  const int ab0 = b.pp_start[P], nab = active ? b.pp_count[P] : 0;
+  const int ab0 = uni(b.pp_start[P]), nab = uni(b.pp_count[P]);
   const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
+  const PrimPairF* __restrict__ ppf = b.ppf;
 
   CT ef[NEL * NFL];
 #pragma unroll
   for (int i = 0; i < NEL * NFL; ++i) ef[i] = CT(0);
 
-  for (int i = 0; i < nab; ++i) {
-    const PrimPair X = b.pp[ab0 + i];
-    const CT PAx = CT(X.Px - Ax), PAy = CT(X.Py - Ay), PAz = CT(X.Pz - Az);
-    const CT p = CT(X.eta), ip = CT(X.inv_eta), ckp = CT(X.cK * X.inv_eta);
-    for (int j = 0; j < ncd; ++j) {
-      const PrimPair Y = b.pp[cd0 + j];
-      const CT q = CT(Y.eta), iq = CT(Y.inv_eta);
-      const CT inv_pq = rcp_ct(p + q);
+  PrimPairF Yn;
+  if (ncd > 0) Yn = ppf[cd0];
+  for (int j = 0; j < ncd; ++j) {
+    const PrimPairF Y = Yn;
+    if (j + 1 < ncd) Yn = ppf[cd0 + j + 1];  // next ket pair, in flight during this one
+    const CT q = CT(Y.eta), iq = CT(Y.inv_eta), ckq = CT(Y.ck);
+    const CT QCx = CT(Y.PAx), QCy = CT(Y.PAy), QCz = CT(Y.PAz);
+    for (int i = 0; i < nab; ++i) {
+      const PrimPairF X = ppf[ab0 + i];  // wave-uniform
+      const CT p = CT(X.eta), ip = CT(X.inv_eta);
+      const CT PAx = CT(X.PAx), PAy = CT(X.PAy), PAz = CT(X.PAz);
+      const CT spq = p + q;
+      const CT inv_pq = rcp_nr(spq);
       const CT dx = CT(X.Px - Y.Px), dy = CT(X.Py - Y.Py), dz = CT(X.Pz - Y.Pz);
       const CT T = p * q * inv_pq * (dx * dx + dy * dy + dz * dz);
-      // 2 pi^(5/2) / (p q sqrt(p+q)) cK_ab cK_cd
-      const CT base = CT(34.98683665524972497) * ckp * CT(Y.cK * Y.inv_eta) * sqrt_ct(inv_pq);
+      // 2 pi^(5/2) cK_ab cK_cd / (p q sqrt(p+q))
+      const CT base = CT(X.ck) * ckq * rsq_nr(spq);
       CT rx[NR], rw[NR];
-      roots_ct<NR>(T, rx, rw);
+      rys_roots_blk<NR, CT>(T, tab, rx, rw);
       const CT qf = q * inv_pq, pf = p * inv_pq;
-      const CT QCx = CT(Y.Px - Cx), QCy = CT(Y.Py - Cy), QCz = CT(Y.Pz - Cz);
 #pragma unroll
       for (int u = 0; u < NR; ++u) {
         const CT x = rx[u];
@@ -347,26 +358,18 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
     for (int ic = 0; ic < NCL; ++ic)
 #pragma unroll
       for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = Dm[(aoC + ic) * N + aoD + id];
-    // The lanes of a row share the bra pair, hence the J[a,b] addresses: when
-    // the whole wave has one bra pair, reduce across lanes and issue one
-    // atomic per element instead of 64 contending ones.
+    // The bra pair, hence the J[a,b] addresses, is the same on every lane:
+    // reduce across the wave and issue one atomic per element.
     const unsigned long long amask = __ballot(active);
     const int first = amask ? (__ffsll((unsigned long long)amask) - 1) : 0;
-    const int Pfirst = __shfl(P, first);
-    const bool uniform = __all(!active || P == Pfirst);
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int r = 0; r < NAL * NB; ++r) {
       double s = 0.0;
 #pragma unroll
       for (int c = 0; c < NCDL; ++c) s = fma((double)tile[r * NCDL + c], dcd[c], s);
-      s = active ? fj * s : 0.0;
-      if (uniform) {
-        s = wave_sum(s);
-        if (lane == first && active) atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
-      } else if (active) {
-        atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
-      }
+      s = wave_sum(active ? fj * s : 0.0);
+      if (lane == first && active) atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
     }
     if (bk && active) {
       double dab[NAL * NB];
@@ -462,75 +465,118 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 
 template <class PL, int CH, typename CT>
 __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp,
-                                               bool active) {
+                                               bool active, const CT* __restrict__ tab) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH, CT>(b, o, P, Qp, active);
+      run_chunk<PL, CH, CT>(b, o, P, Qp, active, tab);
       return;
     }
-    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, active);
+    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, active, tab);
   }
 }
 
-// Work mapping: wavefront g of the grid-stride loop takes 64 consecutive
-// quartets (one per lane) and one chunk, chunk = g mod NCH, so the chunk (and
-// with it all register indexing) is uniform across the wave.
+// Work mapping: wavefront g of the grid-stride loop takes wave unit g / NCH
+// (a bra pair and up to 64 kets of its row, one per lane) and chunk g mod NCH,
+// so the bra pair and the chunk (and with it all register indexing) are
+// uniform across the wave.
 //
-// Precision: with o.mixed a wave votes FP32 when every active quartet's bound
-// Q_ab Q_cd max|D| is below tau. The FP64 instantiation (CT = double) takes
-// the waves that vote FP64 (all waves when o.mixed is off), the FP32
-// instantiation the others; the vote is a pure function of the wave's
-// quartets, so every wave is processed by exactly one of the two launches,
-// each with its own register allocation.
-template <class PL, typename CT>
-__global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o) {
-  constexpr int NCH = PL::NCH;
-  constexpr bool kFP32 = sizeof(CT) == 4;
-  const long long gtid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const long long wave = gtid >> 6;
-  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+// Precision: with o.mixed a wave runs the FP32 instantiation when every active
+// quartet's bound Q_ab Q_cd max|D| is below tau, else the FP64 one; the vote
+// is uniform, so the wave never executes both.
+// The read-only inputs are also passed as __restrict__ arguments and rebound
+// into b, w and o: with no possible aliasing store (the J/K atomics) the
+// compiler can turn wave-uniform loads (wave unit, bra pair, bra primitive
+// pairs, D blocks of the bra) into scalar loads.
+template <class PL>
+__global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o,
+                                                       const PrimPairF* __restrict__ ppf,
+                                                       const WaveUnit* __restrict__ units,
+                                                       const int* __restrict__ col_pair,
+                                                       const int* __restrict__ sp_A, const int* __restrict__ sp_B,
+                                                       const int* __restrict__ pp_start,
+                                                       const int* __restrict__ pp_count,
+                                                       const double* __restrict__ shell_xyz,
+                                                       const int* __restrict__ shell_ao,
+                                                       const double* __restrict__ Qb,
+                                                       const double* __restrict__ Dsp,
+                                                       const double* __restrict__ Dm) {
+  b.ppf = ppf;
+  b.sp_A = sp_A;
+  b.sp_B = sp_B;
+  b.pp_start = pp_start;
+  b.pp_count = pp_count;
+  b.shell_xyz = shell_xyz;
+  b.shell_ao = shell_ao;
+  b.Q = Qb;
+  b.Dsp = Dsp;
+  w.units = units;
+  w.col_pair = col_pair;
+  o.D = Dm;
+  constexpr int NCH = PL::NCH, NR = PL::NR;
+  constexpr bool kLds = NR <= kLdsRootsMax;
+  constexpr int TB = kLds ? rys_c::block_size(NR) : 1;
+  __shared__ double t64[TB];
+  __shared__ float t32[TB];
+  const double* g64 = rys_dev::rys_coef + rys_c::coef_off[NR];
+  const float* g32 = rys_dev_f::rys_coef + rys_c::coef_off[NR];
+  if constexpr (kLds) {
+    for (int t = threadIdx.x; t < TB; t += 256) {
+      t64[t] = g64[t];
+      t32[t] = g32[t];
+    }
+    __syncthreads();
+  }
+  const double* tab64 = kLds ? t64 : g64;
+  const float* tab32 = kLds ? t32 : g32;
   const int lane = threadIdx.x & 63;
-  const long long ngroups = (w.nwork + 63) >> 6;
-  for (long long g = wave; g < ngroups * NCH; g += nwaves) {  // uniform trip count per wave
-    const int chunk = (int)(g % NCH);
-    const long long k = (g / NCH) * 64 + lane;
-    bool active = k < w.nwork;
-    int P = 0, Qp = 0;
-    if (active) active = decode_work(w, k, P, Qp);
+  // launched with 256-thread blocks (4 waves); blockDim is not read because
+  // the compiler cannot prove it uniform, which would make the loop divergent
+  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  const long long nwaves = (long long)gridDim.x * 4;
+  const long long total = w.nunits * NCH;
+  for (long long g = wave; g < total; g += nwaves) {  // uniform trip count per wave
+    const long long u = g / NCH;
+    const int chunk = (int)(g - u * NCH);
+    const WaveUnit un = w.units[u];
+    const int P = uni(un.P);
+    bool active = lane < un.ncol;
+    const int Qp = w.col_pair[un.col + (active ? lane : 0)];
     double bound = 0.0;
     if (active) {
       bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
       if (o.eps_density > 0.0) active = bound >= o.eps_density;
     }
-    const bool lowp = o.mixed && __all(!active || bound < o.tau_mixed);
-    if (lowp != kFP32) continue;  // the other instantiation owns this wave
+    if (!wave_any(active)) continue;  // the whole unit is density-screened
+    const bool lowp = o.mixed && wave_all(!active || bound < o.tau_mixed);
     if (o.counters && chunk == 0 && active) {
       const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
       unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
       const unsigned long long kq = (unsigned long long)b.pp_count[P] * (unsigned long long)b.pp_count[Qp];
       atomicAdd(cnt + (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp)), 1ull);
       atomicAdd(cnt + 8, kq);
-      if (kFP32) {
+      if (lowp) {
         atomicAdd(cnt + 9, 1ull);
         atomicAdd(cnt + 10, kq);
       }
     }
-    // inactive lanes (tail, screened) run with empty primitive loops and no
+    // inactive lanes (tail, screened) run with an empty primitive loop and no
     // atomics so the wave stays converged for the cross-lane reductions
-    dispatch_chunk<PL, 0, CT>(chunk, b, o, P, Qp, active);
+    if (lowp)
+      dispatch_chunk<PL, 0, float>(chunk, b, o, P, Qp, active, tab32);
+    else
+      dispatch_chunk<PL, 0, double>(chunk, b, o, P, Qp, active, tab64);
   }
-  (void)lane;
 }
 
 template <class PL>
 hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {
   const int block = 256;  // 4 waves
-  const long long waves = ((w.nwork + 63) / 64) * PL::NCH;
+  const long long waves = w.nunits * PL::NCH;
   long long g = (waves + 3) / 4;
   if (g > 16384) g = 16384;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL((eri_fast_kernel<PL, double>), dim3((unsigned)g), dim3(block), 0, s, b, w, o);
-  if (o.mixed) hipLaunchKernelGGL((eri_fast_kernel<PL, float>), dim3((unsigned)g), dim3(block), 0, s, b, w, o);
+  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(block), 0, s, b, w, o, b.ppf, w.units,
+                     w.col_pair, b.sp_A, b.sp_B, b.pp_start, b.pp_count, b.shell_xyz, b.shell_ao, b.Q, b.Dsp, o.D);
   return hipGetLastError();
 }
 

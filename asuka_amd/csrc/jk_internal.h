@@ -22,6 +22,17 @@ struct __align__(16) PrimPair {
   double inv_eta;
 };
 
+// The same primitive pair in the form the class-specialised kernels read:
+// everything the primitive loop needs, nothing else. ck folds the Rys
+// prefactor: the product ck_ab ck_cd / sqrt(eta_ab + eta_cd) is
+// 2 pi^(5/2) cK_ab cK_cd / (eta_ab eta_cd sqrt(eta_ab + eta_cd)).
+struct __align__(16) PrimPairF {
+  double eta, inv_eta, ck;
+  double Px, Py, Pz;
+  double PAx, PAy, PAz;  // P minus the centre of the pair's first shell
+  double pad;
+};
+
 enum EriMode : int {
   MODE_JK = 0,       // accumulate J and/or K
   MODE_FOCK = 1,     // accumulate J - K/2 into F
@@ -39,6 +50,7 @@ struct DevBasis {
   const int* pp_start;       // (nsp) first primitive pair of the shell pair
   const int* pp_count;       // (nsp)
   const PrimPair* pp;
+  const PrimPairF* ppf;      // same indexing as pp
   const double* Q;           // (nsp) Schwarz bounds
   const double* Dsp;         // (nsp) max |D| per shell-pair block
   int nao;
@@ -62,8 +74,19 @@ struct Segment {
   int pad;
 };
 
+// One wavefront's share of a segment: a bra pair and up to 64 consecutive
+// kets of its row (one per lane), so the bra side is uniform across the wave.
+struct WaveUnit {
+  int P;     // bra shell pair
+  int col;   // first ket, index into col_pair
+  int ncol;  // 1..64
+  int pad;
+};
+
 struct ClassWork {
   int la, lb, lc, ld;
+  const WaveUnit* units;  // class-specialised kernels (bra-uniform wavefronts)
+  long long nunits;
   const Segment* seg;
   int nseg;
   const int* row_pair;

@@ -124,3 +124,70 @@ __device__ inline void rys_roots_dev_f(float T, float* x, float* w) {
       w[f - N] = a;
   }
 }
+
+// Compile-time copies of the table metadata.
+namespace rys_c {
+constexpr int nint[] = {RYS_NINT_LIST};
+constexpr double tmax[] = {RYS_TMAX_LIST};
+constexpr int coef_off[] = {RYS_COEF_OFF_LIST};
+// coefficients of the n-root block
+constexpr int block_size(int n) { return nint[n] * 2 * n * (RYS_DEG + 1); }
+static_assert(RYS_H == 2.0, "the block evaluators assume intervals of width 2");
+}  // namespace rys_c
+
+// Reciprocal and reciprocal square root in FP64: the hardware estimate
+// (v_rcp_f64 / v_rsq_f64) refined by two Newton-Raphson steps, each of which
+// squares the relative error, to within an ulp or two of the correctly
+// rounded result. Valid for normal positive arguments far from the
+// overflow/underflow limits (pair exponents and Boys arguments are).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double t = fma(-h * y, y, 0.5);  // y <- y (3 - x y^2) / 2
+  y = fma(y, t, y);
+  t = fma(-h * y, y, 0.5);
+  return fma(y, t, y);
+}
+__device__ __forceinline__ float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_nr(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// All N roots and weights at T from the N-root coefficient block `blk`
+// (rys_coef + coef_off[N], or a copy of it in LDS), in the arithmetic of CT.
+// At T >= tmax[N] the Gauss-Hermite limit, with 1/T and 1/sqrt(T) in CT.
+template <int N, typename CT>
+__device__ __forceinline__ void rys_roots_blk(CT T, const CT* __restrict__ blk, CT* x, CT* w) {
+  constexpr int NI = rys_c::nint[N];
+  constexpr int NF = 2 * N * (RYS_DEG + 1);
+  if (T >= CT(rys_c::tmax[N])) {
+    const CT it = rcp_nr(T);
+    const CT sit = rsq_nr(T);
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      x[u] = CT(rys_dev::rys_herm_x[rys_dev::rys_herm_off[N] + u]) * it;
+      w[u] = CT(rys_dev::rys_herm_w[rys_dev::rys_herm_off[N] + u]) * sit;
+    }
+    return;
+  }
+  int i = (int)(T * CT(0.5));
+  i = i < NI - 1 ? i : NI - 1;
+  const CT s = (T - CT(2 * i)) - CT(1);
+  const CT* c = blk + i * NF;
+#pragma unroll
+  for (int f = 0; f < 2 * N; ++f) {
+    const CT* cf = c + f * (RYS_DEG + 1);
+    CT a = cf[RYS_DEG];
+#pragma unroll
+    for (int d = RYS_DEG - 1; d >= 0; --d) a = fma(a, s, cf[d]);
+    if (f < N)
+      x[f] = a;
+    else
+      w[f - N] = a;
+  }
+}

@@ -62,6 +62,7 @@ class DirectJKContext:
     Q_sp: np.ndarray
     natm: int = 0
     nprim_pairs: int = 0
+    atom_coords_bohr: np.ndarray | None = field(default=None, repr=False)
     _keep: tuple = field(default=(), repr=False)
 
     def __del__(self):
@@ -167,7 +168,10 @@ def make_direct_jk_context(
     return DirectJKContext(ao_basis=ao_basis, nao=int(info[0]), max_l=int(info[4]), nsp=nsp, ntasks=int(info[3]),
                            eps_schwarz=float(eps_schwarz), eps_density=float(eps_density), device=device,
                            handle=int(h.value), sp_A=sp_A, sp_B=sp_B, Q_sp=Q, natm=int(info[6]),
-                           nprim_pairs=int(info[7]), _keep=tuple(keep))
+                           nprim_pairs=int(info[7]),
+                           atom_coords_bohr=(None if atom_coords_bohr is None
+                                             else np.array(atom_coords_bohr, dtype=np.float64).reshape(-1, 3)),
+                           _keep=tuple(keep))
 
 
 def _as_device_matrix(torch, X, nao: int, name: str, device):

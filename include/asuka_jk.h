@@ -142,6 +142,27 @@ int asuka_jk_eri_tiles(asuka_jk_ctx* ctx, int64_t nq, const int32_t* spab, const
  * (asuka/integrals/int1e_cart.py:1423-1434). */
 int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream);
 
+/* Backward pass of the J/K build with respect to the nuclear coordinates:
+ * grad (natm x 3, device, overwritten) = dL/dR for
+ *   L = Tr(WJ J(D)) + Tr(WK K(D))
+ * at fixed D, WJ, WK (nao x nao, symmetric, device; WJ or WK may be NULL),
+ * over the context's Schwarz-screened quartets. For the RHF two-electron
+ * energy pass WJ = D/2, WK = -D/4. Needs atoms (natm > 0) and the shell->atom
+ * map. Replaces the 4c derivative contraction of the reference:
+ * asuka/mcscf/nac/_dense.py:1513 grad_2e_ham_dense_eri4c_contracted with the
+ * adjoint tile of :195 _bar_tile_meanfield_jk, and the kernel
+ * asuka/cueri/_eri_rys_cpu.pyx:4316 (CUDA: cueri/gpu.py:6804). */
+int asuka_jk_grad(asuka_jk_ctx* ctx, const double* D, const double* WJ, const double* WK, double* grad,
+                  void* stream);
+
+/* One-electron part of the nuclear gradient: grad (natm x 3, device,
+ * overwritten) = dL1/dR for L1 = Tr(Dh (T + V)) - Tr(Ws S) at fixed symmetric
+ * Dh, Ws (either may be NULL), including the nuclear-attraction operator
+ * derivative. Replaces contract_dhcore_cart (asuka/integrals/int1e_cart.py:1738)
+ * and contract_dS_ip_cart (:1545) as assembled in
+ * asuka/mcscf/nuc_grad_direct.py:151-181. */
+int asuka_int1e_grad(asuka_jk_ctx* ctx, const double* Dh, const double* Ws, double* grad, void* stream);
+
 /* Rys roots (x = t^2, ascending) and weights of the n-point rule at T, from the
  * interpolation table the kernels use, evaluated on the host (n <= 7). */
 int asuka_rys_roots(int n, double T, double* x, double* w);

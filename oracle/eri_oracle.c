@@ -668,3 +668,175 @@ void oracle_dense_eri(const oracle_basis* bs, int nao, int hp, int nthreads, dou
     free(tile);
   }
 }
+
+/* ---------------- nuclear derivatives (pyx:4316-4656) ----------------
+ * out[center*3 + xyz] += sum_{abcd} bar[(ia*nB+ib)*nCD + ic*nD+id] d(ab|cd)/dR_center,xyz
+ * for the four centres [A, B, C, D], each from its own primitive exponent:
+ * d/dA_x = 2 alpha (a+1_x ..) - a_x (a-1_x ..). Roots floor((L+1)/2)+1, tables to
+ * la+lb+1 and lc+ld+1. The pair index decodes to primitives as ia = idx / nprimB. */
+static void deriv_contract_sp(const oracle_basis* bs, int spAB, int spCD, int hp, const double* bar, double* out) {
+  int A = bs->sp_A[spAB], B = bs->sp_B[spAB], C = bs->sp_A[spCD], D = bs->sp_B[spCD];
+  int la = bs->shell_l[A], lb = bs->shell_l[B], lc = bs->shell_l[C], ld = bs->shell_l[D];
+  int nA = ncart(la), nB = ncart(lb), nC = ncart(lc), nD = ncart(ld), nCD = nC * nD;
+  const double* cA = bs->shell_cxyz + 3 * A;
+  const double* cB = bs->shell_cxyz + 3 * B;
+  const double* cC = bs->shell_cxyz + 3 * C;
+  const double* cD = bs->shell_cxyz + 3 * D;
+  double xij[3][LMAX + 3], xkl[3][LMAX + 3];
+  for (int ax = 0; ax < 3; ++ax) {
+    xij[ax][0] = xkl[ax][0] = 1.0;
+    for (int p = 1; p <= LMAX + 2; ++p) {
+      xij[ax][p] = xij[ax][p - 1] * (cA[ax] - cB[ax]);
+      xkl[ax][p] = xkl[ax][p - 1] * (cC[ax] - cD[ax]);
+    }
+  }
+  int e[4][3][NCART_MAX];  /* Cartesian exponents per centre, axis, component */
+  cart_comp(la, e[0][0], e[0][1], e[0][2]);
+  cart_comp(lb, e[1][0], e[1][1], e[1][2]);
+  cart_comp(lc, e[2][0], e[2][1], e[2][2]);
+  cart_comp(ld, e[3][0], e[3][1], e[3][2]);
+  int nmax = la + lb + 1, mmax = lc + ld + 1, nroots = (la + lb + lc + ld + 1) / 2 + 1;
+  int baseAB = bs->sp_pair_start[spAB], baseCD = bs->sp_pair_start[spCD];
+  int nAB_p = bs->sp_pair_start[spAB + 1] - baseAB, nCD_p = bs->sp_pair_start[spCD + 1] - baseCD;
+  int nprimB = bs->shell_nprim[B], nprimD = bs->shell_nprim[D];
+  double roots[NROOTS_MAX], weights[NROOTS_MAX];
+  double G[3][GSIZE];
+  for (int iab = 0; iab < nAB_p; ++iab) {
+    double p = bs->pair_eta[baseAB + iab];
+    double P[3] = {bs->pair_Px[baseAB + iab], bs->pair_Py[baseAB + iab], bs->pair_Pz[baseAB + iab]};
+    double cKab = bs->pair_cK[baseAB + iab];
+    int ipA = iab / nprimB, ipB = iab - ipA * nprimB;
+    double ex[4];
+    ex[0] = bs->prim_exp[bs->shell_prim_start[A] + ipA];
+    ex[1] = bs->prim_exp[bs->shell_prim_start[B] + ipB];
+    for (int icd = 0; icd < nCD_p; ++icd) {
+      double q = bs->pair_eta[baseCD + icd];
+      double Q[3] = {bs->pair_Px[baseCD + icd], bs->pair_Py[baseCD + icd], bs->pair_Pz[baseCD + icd]};
+      double cKcd = bs->pair_cK[baseCD + icd];
+      int ipC = icd / nprimD, ipD = icd - ipC * nprimD;
+      ex[2] = bs->prim_exp[bs->shell_prim_start[C] + ipC];
+      ex[3] = bs->prim_exp[bs->shell_prim_start[D] + ipD];
+      double denom = p + q, inv_denom = 1.0 / denom;
+      double PQ2 = 0.0;
+      for (int ax = 0; ax < 3; ++ax) PQ2 += (P[ax] - Q[ax]) * (P[ax] - Q[ax]);
+      double T = p * q * inv_denom * PQ2;
+      double base = kTwoPi52 / (p * q * sqrt(denom)) * cKab * cKcd;
+      oracle_rys(nroots, T, hp, roots, weights);
+      for (int u = 0; u < nroots; ++u) {
+        double x = roots[u];
+        double B0 = x * 0.5 * inv_denom;
+        double B1 = (1.0 - x) * 0.5 / p + B0;
+        double B1p = (1.0 - x) * 0.5 / q + B0;
+        for (int ax = 0; ax < 3; ++ax) {
+          double Cc = (P[ax] - cA[ax]) + (q * inv_denom) * x * (Q[ax] - P[ax]);
+          double Cp = (Q[ax] - cC[ax]) + (p * inv_denom) * x * (P[ax] - Q[ax]);
+          compute_G(G[ax], nmax, mmax, Cc, Cp, B0, B1, B1p);
+        }
+        double scale = base * weights[u];
+        for (int ia = 0; ia < nA; ++ia)
+          for (int ib = 0; ib < nB; ++ib)
+            for (int ic = 0; ic < nC; ++ic)
+              for (int id = 0; id < nD; ++id) {
+                double bv = bar[(ia * nB + ib) * nCD + ic * nD + id];
+                if (bv == 0.0) continue;
+                int comp[4] = {ia, ib, ic, id};
+                double I[3];
+                int n4[3][4];
+                for (int ax = 0; ax < 3; ++ax) {
+                  for (int c = 0; c < 4; ++c) n4[ax][c] = e[c][ax][comp[c]];
+                  I[ax] = shift_from_G(G[ax], n4[ax][0], n4[ax][1], n4[ax][2], n4[ax][3], xij[ax], xkl[ax]);
+                }
+                for (int c = 0; c < 4; ++c)
+                  for (int ax = 0; ax < 3; ++ax) {
+                    int nn[4] = {n4[ax][0], n4[ax][1], n4[ax][2], n4[ax][3]};
+                    nn[c] += 1;
+                    double up = shift_from_G(G[ax], nn[0], nn[1], nn[2], nn[3], xij[ax], xkl[ax]);
+                    double dn = 0.0;
+                    if (n4[ax][c] > 0) {
+                      nn[c] -= 2;
+                      dn = shift_from_G(G[ax], nn[0], nn[1], nn[2], nn[3], xij[ax], xkl[ax]);
+                    }
+                    double dI = 2.0 * ex[c] * up - (double)n4[ax][c] * dn;
+                    double other = I[(ax + 1) % 3] * I[(ax + 2) % 3];
+                    out[c * 3 + ax] += bv * scale * dI * other;
+                  }
+              }
+      }
+    }
+  }
+}
+
+/* Adjoint tile for Tr(DwJ J(Dc)) - 1/2 Tr(DwK K(Dc)) of one ordered pair of
+ * shell pairs, in (AB|CD) layout with the pair folding of
+ * asuka/mcscf/nac/_dense.py:195-250 (there DwJ = DwK). */
+static void bar_tile(const oracle_basis* bs, int spAB, int spCD, int nao, const double* DwJ, const double* DwK,
+                     const double* Dc, double* bar) {
+  int A = bs->sp_A[spAB], B = bs->sp_B[spAB], C = bs->sp_A[spCD], D = bs->sp_B[spCD];
+  int nA = ncart(bs->shell_l[A]), nB = ncart(bs->shell_l[B]), nC = ncart(bs->shell_l[C]), nD = ncart(bs->shell_l[D]);
+  long a0 = bs->shell_ao_start[A], b0 = bs->shell_ao_start[B], c0 = bs->shell_ao_start[C], d0 = bs->shell_ao_start[D];
+  int same_ab = A == B, same_cd = C == D;
+  long N = nao;
+  for (int ia = 0; ia < nA; ++ia)
+    for (int ib = 0; ib < nB; ++ib)
+      for (int ic = 0; ic < nC; ++ic)
+        for (int id = 0; id < nD; ++id) {
+          long a = a0 + ia, b = b0 + ib, c = c0 + ic, d = d0 + id;
+          double v = 0.0;
+          if (DwJ) {
+            double wab = DwJ[a * N + b] + (same_ab ? 0.0 : DwJ[b * N + a]);
+            double dcd = Dc[c * N + d] + (same_cd ? 0.0 : Dc[d * N + c]);
+            v += wab * dcd;
+          }
+          if (DwK) {
+            double t = DwK[a * N + c] * Dc[b * N + d];
+            if (!same_ab) t += DwK[b * N + c] * Dc[a * N + d];
+            if (!same_cd) t += DwK[a * N + d] * Dc[b * N + c];
+            if (!same_ab && !same_cd) t += DwK[b * N + d] * Dc[a * N + c];
+            v += -0.5 * t;
+          }
+          bar[(ia * nB + ib) * nC * nD + ic * nD + id] = v;
+        }
+}
+
+/* Per-quartet derivative contraction with a caller-given adjoint tile. */
+int oracle_eri_deriv(const oracle_basis* bs, int spAB, int spCD, int hp, const double* bar, double* out12) {
+  memset(out12, 0, sizeof(double) * 12);
+  deriv_contract_sp(bs, spAB, spCD, hp, bar, out12);
+  return 0;
+}
+
+/* Two-electron gradient of Tr(DwJ J(Dc)) - 1/2 Tr(DwK K(Dc)) with the
+ * reference's driver loop (asuka/mcscf/nac/_dense.py:1513-1839): every ordered
+ * pair (spAB, spCD) of shell pairs, no screening, per-quartet (4, 3) output
+ * scattered to the atoms of A, B, C, D. grad (natm, 3) is overwritten. */
+void oracle_grad2e(const oracle_basis* bs, int nao, const double* DwJ, const double* DwK, const double* Dc,
+                   const int32_t* shell_atom, int natm, int hp, int nthreads, double* grad) {
+  memset(grad, 0, sizeof(double) * 3 * natm);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    double* bar = (double*)malloc(sizeof(double) * NCART_MAX * NCART_MAX * NCART_MAX * NCART_MAX);
+    double* gp = (double*)calloc(3 * (size_t)natm, sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int ab = 0; ab < bs->nsp; ++ab)
+      for (int cd = 0; cd < bs->nsp; ++cd) {
+        double out[12] = {0};
+        bar_tile(bs, ab, cd, nao, DwJ, DwK, Dc, bar);
+        deriv_contract_sp(bs, ab, cd, hp, bar, out);
+        int at[4] = {shell_atom[bs->sp_A[ab]], shell_atom[bs->sp_B[ab]], shell_atom[bs->sp_A[cd]],
+                     shell_atom[bs->sp_B[cd]]};
+        for (int c = 0; c < 4; ++c)
+          for (int ax = 0; ax < 3; ++ax) gp[3 * at[c] + ax] += out[3 * c + ax];
+      }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (int i = 0; i < 3 * natm; ++i) grad[i] += gp[i];
+    free(bar);
+    free(gp);
+  }
+}

@@ -72,6 +72,11 @@ def lib():
         L.oracle_int1e.argtypes = [ctypes.c_int, _f64p, _i32p, _i32p, _i32p, _i32p, _f64p, _f64p, ctypes.c_int,
                                    _f64p, _f64p, ctypes.c_int, _f64p, _f64p, _f64p]
         L.oracle_int1e.restype = ctypes.c_int
+        L.oracle_eri_deriv.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _f64p, _f64p]
+        L.oracle_eri_deriv.restype = ctypes.c_int
+        L.oracle_grad2e.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, _f64p, _f64p, _f64p, _i32p,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
         _LIB = L
     return _LIB
 
@@ -290,6 +295,37 @@ def dense_JK(eri_mat: np.ndarray, D: np.ndarray):
     J = np.einsum("mnls,ls->mn", e4, D, optimize=True)
     K = np.einsum("mlns,ls->mn", e4, D, optimize=True)
     return 0.5 * (J + J.T), 0.5 * (K + K.T)
+
+
+# ---------------------------------------------------------------------------
+# Nuclear derivatives of the ERIs
+# ---------------------------------------------------------------------------
+
+
+def eri_deriv(ctx: OracleContext, spAB: int, spCD: int, bar: np.ndarray, hp: bool = True) -> np.ndarray:
+    """(4, 3) = sum bar * d(ab|cd)/dR_center for centres [A, B, C, D]
+    (_eri_rys_cpu.pyx:4316-4656)."""
+    bar = np.ascontiguousarray(bar, dtype=np.float64)
+    out = np.zeros(12)
+    lib().oracle_eri_deriv(ctypes.byref(ctx.struct), int(spAB), int(spCD), int(bool(hp)), _p(bar, _f64p),
+                           _p(out, _f64p))
+    return out.reshape(4, 3)
+
+
+def grad2e(ctx: OracleContext, D: np.ndarray, WJ: np.ndarray | None, WK: np.ndarray | None,
+           shell_atom: np.ndarray, natm: int, hp: bool = True, threads: int = 0) -> np.ndarray:
+    """d/dR [Tr(WJ J(D)) + Tr(WK K(D))] at fixed D, WJ, WK with the reference's
+    driver (mcscf/nac/_dense.py:1513-1839: all ordered shell-pair pairs, adjoint
+    tile of :195-250 with D_w -> WJ for the Coulomb part and -2 WK for the
+    exchange part, D_core -> D)."""
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    wj = None if WJ is None else np.ascontiguousarray(WJ, dtype=np.float64)
+    wk = None if WK is None else np.ascontiguousarray(-2.0 * np.asarray(WK), dtype=np.float64)
+    sa = np.ascontiguousarray(shell_atom, dtype=np.int32)
+    g = np.zeros((int(natm), 3))
+    lib().oracle_grad2e(ctypes.byref(ctx.struct), ctx.nao, _p(wj, _f64p), _p(wk, _f64p), _p(D, _f64p),
+                        _p(sa, _i32p), int(natm), int(bool(hp)), int(threads), _p(g, _f64p))
+    return g
 
 
 # ---------------------------------------------------------------------------
