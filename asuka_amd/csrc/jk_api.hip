@@ -30,6 +30,10 @@ static thread_local std::string g_err;
 
 // default FP32 threshold on Q_ab Q_cd max|D| for builds with ASUKA_JK_MIXED
 static constexpr double kDefaultTauMixed = 1e-7;
+// Row tasks: at most this many wave units of one bra row per workgroup.
+static constexpr int kTaskUnits = 16;
+// LDS accumulators of one row task larger than this fall back to global atomics.
+static constexpr size_t kRowAccMaxBytes = 48 * 1024;
 // sqrt(2 pi^(5/2)), split evenly over the bra and ket primitive-pair factors
 static constexpr double kSqrt2Pi52 = 5.91496717279561287782;
 
@@ -54,6 +58,8 @@ struct ClassPairHost {
   int seg_start, nseg;
   long long nwork;      // padded work index range
   long long unit_start, nunits;  // bra-uniform wavefront units
+  long long task_start, ntasks;  // row tasks (workgroups)
+  int acc_colsC, acc_colsD;      // LDS row accumulators (0: direct atomics)
   long long nquartets;  // Schwarz-screened quartets
   long long nprim_quartets;
   long long nprim_roots;
@@ -62,6 +68,11 @@ struct ClassPairHost {
 __global__ void k_fill(double* x, long long n, double v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     x[i] = v;
+}
+
+__global__ void k_zero_u32(unsigned int* x, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = 0u;
 }
 
 __global__ void k_zero_u64(unsigned long long* x, long long n) {
@@ -150,6 +161,9 @@ struct asuka_jk_ctx {
   int* d_col_pair = nullptr;
   Segment* d_segs = nullptr;
   WaveUnit* d_units = nullptr;
+  RowTask* d_tasks = nullptr;
+  int lao_off[ASUKA_LMAX + 2] = {0};
+  int nlao[ASUKA_LMAX + 1] = {0};
   double* d_Q = nullptr;
   double* d_Dsp = nullptr;
   double* d_atom_xyz = nullptr;
@@ -160,6 +174,7 @@ struct asuka_jk_ctx {
   double* d_prim_exp = nullptr;
   double* d_prim_coef = nullptr;
   unsigned long long* d_counters = nullptr;
+  unsigned int* d_queue = nullptr;  // one work-queue head per class pair
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   long long last_launches = 0;
   std::vector<void*> owned;
@@ -293,6 +308,25 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
     if ((rc = dev_upload(&ctx->d_prim_exp, pe, own, s))) return rc;
     if ((rc = dev_upload(&ctx->d_prim_coef, pc, own, s))) return rc;
   }
+  // compact AO numbering per angular momentum (the LDS accumulator columns)
+  {
+    std::vector<int> shell_lao(nshell), lao_ao;
+    for (int l = 0; l <= ASUKA_LMAX; ++l) {
+      ctx->lao_off[l] = (int)lao_ao.size();
+      for (int sh = 0; sh < nshell; ++sh)
+        if (ctx->shell_l[sh] == l) {
+          shell_lao[sh] = (int)lao_ao.size() - ctx->lao_off[l];
+          for (int c = 0; c < ncart_c(l); ++c) lao_ao.push_back(ctx->shell_ao[sh] + c);
+        }
+      ctx->nlao[l] = (int)lao_ao.size() - ctx->lao_off[l];
+    }
+    ctx->lao_off[ASUKA_LMAX + 1] = (int)lao_ao.size();
+    int *d_slao, *d_laoao;
+    if ((rc = dev_upload(&d_slao, shell_lao, own, s))) return rc;
+    if ((rc = dev_upload(&d_laoao, lao_ao, own, s))) return rc;
+    ctx->dev.shell_lao = d_slao;
+    ctx->dev.lao_ao = d_laoao;
+  }
   std::vector<double> zeros(ctx->nsp, 0.0);
   if ((rc = dev_upload(&ctx->d_Q, zeros, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_Dsp, zeros, own, s))) return rc;
@@ -369,6 +403,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   std::vector<long long> row_off_all;
   std::vector<Segment> segs;
   std::vector<WaveUnit> units;
+  std::vector<RowTask> tasks;
   for (int c = 0; c < ncls; ++c) {
     std::vector<int> kvals;
     for (int sp : cls_pairs[c]) kvals.push_back(ctx->pp_count[sp]);
@@ -399,6 +434,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       cp.ld = cls_lb[Y];
       cp.seg_start = (int)segs.size();
       cp.unit_start = (long long)units.size();
+      cp.task_start = (long long)tasks.size();
       const int nroots = (cp.la + cp.lb + cp.lc + cp.ld) / 2 + 1;
       long long padded = 0;
       for (const PairGroup& bg : groups[X]) {
@@ -424,8 +460,17 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
             rowp.push_back(P);
             offs.push_back(off);
             off += len;
-            for (long long c0 = 0; c0 < len; c0 += 64)
-              units.push_back(WaveUnit{P, kg.col_start + (int)c0, (int)std::min<long long>(64, len - c0), 0});
+            {
+              // the row's wave units, in row tasks of at most kTaskUnits
+              const int nu = (int)((len + 63) / 64);
+              const int ntk = (nu + kTaskUnits - 1) / kTaskUnits;
+              for (int k = 0; k < ntk; ++k) {  // balanced task sizes
+                const int b0 = (int)((long long)nu * k / ntk), b1 = (int)((long long)nu * (k + 1) / ntk);
+                tasks.push_back(RowTask{(int)(units.size() - cp.unit_start) + b0, b1 - b0});
+              }
+              for (long long c0 = 0; c0 < len; c0 += 64)
+                units.push_back(WaveUnit{P, kg.col_start + (int)c0, (int)std::min<long long>(64, len - c0), 0});
+            }
           }
           if (off == 0) continue;
           Segment sg{};
@@ -448,6 +493,24 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       }
       cp.nseg = (int)segs.size() - cp.seg_start;
       cp.nunits = (long long)units.size() - cp.unit_start;
+      cp.ntasks = (long long)tasks.size() - cp.task_start;
+      // longest tasks first: the workgroups pull them from a queue, so the
+      // tail of the launch is made of short tasks
+      std::stable_sort(tasks.begin() + cp.task_start, tasks.end(), [&](const RowTask& x, const RowTask& y) {
+        const WaveUnit& ux = units[cp.unit_start + x.u0];
+        const WaveUnit& uy = units[cp.unit_start + y.u0];
+        const long long cx = (long long)x.nu * ctx->pp_count[ux.P] * ctx->pp_count[col_all[ux.col]];
+        const long long cy = (long long)y.nu * ctx->pp_count[uy.P] * ctx->pp_count[col_all[uy.col]];
+        return cx > cy;
+      });
+      {
+        const int cC = ctx->nlao[cp.lc], cD = (cp.lc == cp.ld) ? 0 : ctx->nlao[cp.ld];
+        const size_t bytes = sizeof(double) * (size_t)(ncart_c(cp.la) * ncart_c(cp.lb) +
+                                                       (ncart_c(cp.la) + ncart_c(cp.lb)) * (cC + cD));
+        const bool fits = bytes <= kRowAccMaxBytes;
+        cp.acc_colsC = fits ? cC : 0;
+        cp.acc_colsD = fits ? cD : 0;
+      }
       if (cp.nseg == 0) continue;
       cp.nwork = padded;
       cp.nprim_roots = cp.nprim_quartets * nroots;
@@ -457,12 +520,15 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   }
   if ((rc = dev_upload(&ctx->d_segs, segs, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_units, units, own, s))) return rc;
+  if ((rc = dev_upload(&ctx->d_tasks, tasks, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_pair, row_pair_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_off, row_off_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_col_pair, col_all, own, s))) return rc;
   {
     std::vector<unsigned long long> cz(kCountStride * std::max<size_t>(1, ctx->cps.size()), 0ull);
     if ((rc = dev_upload(&ctx->d_counters, cz, own, s))) return rc;
+    std::vector<unsigned int> qz(std::max<size_t>(1, ctx->cps.size()), 0u);
+    if ((rc = dev_upload(&ctx->d_queue, qz, own, s))) return rc;
   }
   HIP_CHECK(hipStreamSynchronize(s));
   *out = guard.release();
@@ -535,6 +601,13 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   w.nwork = cp.nwork;
   w.units = ctx->d_units + cp.unit_start;
   w.nunits = cp.nunits;
+  w.tasks = ctx->d_tasks + cp.task_start;
+  w.ntasks = cp.ntasks;
+  w.acc_colsC = cp.acc_colsC;
+  w.acc_colsD = cp.acc_colsD;
+  w.lao_offC = ctx->lao_off[cp.lc];
+  w.lao_offD = ctx->lao_off[cp.ld];
+  w.queue = ctx->d_queue + index;
   return w;
 }
 
@@ -544,11 +617,17 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     if (!ctx->ev_stop) HIP_CHECK(hipEventCreate(&ctx->ev_stop));
     HIP_CHECK(hipEventRecord(ctx->ev_start, s));
   }
+  {  // work-queue heads start at zero (a kernel, not a memset: the build may be graph-captured)
+    const long long nq = (long long)std::max<size_t>(1, ctx->cps.size());
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ctx->d_queue, nq);
+    HIP_CHECK(hipGetLastError());
+  }
   long long n = 0;
   for (size_t i = 0; i < ctx->cps.size(); ++i) {
     ClassWork w = work_of(ctx, ctx->cps[i], (int)i);
     bool handled = false;
     HIP_CHECK(launch_eri_fast(ctx->dev, w, o, s, &handled));
+    if (!handled) HIP_CHECK(launch_eri_coop(ctx->dev, w, o, s, &handled));
     if (!handled) HIP_CHECK(launch_eri_generic(ctx->dev, w, o, s));
     ++n;
   }

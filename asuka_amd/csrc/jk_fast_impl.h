@@ -179,6 +179,22 @@ __device__ __forceinline__ float fmac(float a, float b, float c) { return __buil
 // Root tables of up to this many roots are copied to LDS by each workgroup.
 constexpr int kLdsRootsMax = 3;
 
+// Workgroup accumulators of one row task (one bra pair P), in the dynamic
+// LDS array s_acc: the J[a,b] block and the K rows of the bra's AOs, K[a, :]
+// and K[b, :], over the columns of the ket class (AOs of angular momentum lc,
+// then ld, in the compact per-l numbering). Offsets are in doubles; on ==
+// false selects direct global atomics.
+extern __shared__ double s_acc[];
+struct RowAcc {
+  bool on;
+  int jab;  // [NA * NB]
+  int kac;  // [NA * colsC]
+  int kad;  // [NA * colsD] (same as kac when lc == ld)
+  int kbc;  // [NB * colsC]
+  int kbd;  // [NB * colsD] (same as kbc when lc == ld)
+  int colsC, colsD;
+};
+
 // One chunk of one shell quartet per lane; the bra pair P is the same on every
 // lane of the wave (bra-uniform wave units), so the bra primitive pairs are
 // read with scalar loads in the inner loop while each lane streams its own
@@ -188,7 +204,7 @@ constexpr int kLdsRootsMax = 3;
 // `tab` is the NR-root block of the root table in CT (LDS or global).
 template <class PL, int CH, typename CT>
 __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, bool active,
-                                          const CT* __restrict__ tab) {
+                                          const CT* __restrict__ tab, const RowAcc& acc) {
   using CK = Chunk<PL, CH>;
   constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
   constexpr int NMAX = PL::NMAX, MMAX = PL::MMAX;
@@ -342,7 +358,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 
   // ---- digestion of the chunk's tile block (rows a0..a1 x all b, cols c0..c1 x all d) ----
   // Sums over the chunk's own components are complete; the rest are partial
-  // and the atomics add them up across chunks.
+  // and add up across chunks. J[a,b] and the K rows of a and b go to the
+  // workgroup's LDS accumulators when present (flushed once per row task),
+  // J[c,d] (distinct per lane) to global atomics.
   const long long N = b.nao;
   const int aoA = b.shell_ao[A] + a0, aoB = b.shell_ao[B], aoC = b.shell_ao[C] + c0, aoD = b.shell_ao[Dsh];
   const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
@@ -352,6 +370,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
   double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
   const double ks = (o.mode == MODE_FOCK) ? -0.5 * fk : fk;
+  const bool lds = acc.on;  // uniform
+  // compact columns of the lane's C (first component of the chunk) and D shells
+  const int colC = lds ? b.shell_lao[C] + c0 : 0, colD = lds ? b.shell_lao[Dsh] : 0;
   if (Jo) {
     double dcd[NCDL];
 #pragma unroll
@@ -359,7 +380,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
       for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = Dm[(aoC + ic) * N + aoD + id];
     // The bra pair, hence the J[a,b] addresses, is the same on every lane:
-    // reduce across the wave and issue one atomic per element.
+    // reduce across the wave, one update per element.
     const unsigned long long amask = __ballot(active);
     const int first = amask ? (__ffsll((unsigned long long)amask) - 1) : 0;
     const int lane = threadIdx.x & 63;
@@ -369,7 +390,12 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
       for (int c = 0; c < NCDL; ++c) s = fma((double)tile[r * NCDL + c], dcd[c], s);
       s = wave_sum(active ? fj * s : 0.0);
-      if (lane == first && active) atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
+      if (lane == first && active) {
+        if (lds)
+          atomicAdd(&s_acc[acc.jab + (a0 + r / NB) * NB + r % NB], s);
+        else
+          atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
+      }
     }
     if (bk && active) {
       double dab[NAL * NB];
@@ -402,7 +428,10 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
           for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
             for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbd[ib * ND + id], s);
-          atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], ks * s);
+          if (lds)
+            atomicAdd(&s_acc[acc.kac + (a0 + ia) * acc.colsC + colC + ic], ks * s);
+          else
+            atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], ks * s);
         }
     }
     if (cd_neq) {  // K[a,d] += sum_{b,c} v D[b,c]
@@ -420,7 +449,10 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
           for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
             for (int ic = 0; ic < NCL; ++ic) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbc[ib * NCL + ic], s);
-          atomicAdd(&Ko[(aoA + ia) * N + aoD + id], ks * s);
+          if (lds)
+            atomicAdd(&s_acc[acc.kad + (a0 + ia) * acc.colsD + colD + id], ks * s);
+          else
+            atomicAdd(&Ko[(aoA + ia) * N + aoD + id], ks * s);
         }
     }
     if (ab_neq) {  // K[b,c] += sum_{a,d} v D[a,d]
@@ -438,7 +470,10 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
           for (int ia = 0; ia < NAL; ++ia)
 #pragma unroll
             for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dad[ia * ND + id], s);
-          atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], ks * s);
+          if (lds)
+            atomicAdd(&s_acc[acc.kbc + ib * acc.colsC + colC + ic], ks * s);
+          else
+            atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], ks * s);
         }
       if (cd_neq) {  // K[b,d] += sum_{a,c} v D[a,c]
         double dac[NAL * NCL];
@@ -456,7 +491,10 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
               for (int ic = 0; ic < NCL; ++ic)
                 s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dac[ia * NCL + ic], s);
-            atomicAdd(&Ko[(aoB + ib) * N + aoD + id], ks * s);
+            if (lds)
+              atomicAdd(&s_acc[acc.kbd + ib * acc.colsD + colD + id], ks * s);
+            else
+              atomicAdd(&Ko[(aoB + ib) * N + aoD + id], ks * s);
           }
       }
     }
@@ -465,24 +503,27 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 
 template <class PL, int CH, typename CT>
 __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp,
-                                               bool active, const CT* __restrict__ tab) {
+                                               bool active, const CT* __restrict__ tab, const RowAcc& acc) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH, CT>(b, o, P, Qp, active, tab);
+      run_chunk<PL, CH, CT>(b, o, P, Qp, active, tab, acc);
       return;
     }
-    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, active, tab);
+    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, active, tab, acc);
   }
 }
 
-// Work mapping: wavefront g of the grid-stride loop takes wave unit g / NCH
-// (a bra pair and up to 64 kets of its row, one per lane) and chunk g mod NCH,
-// so the bra pair and the chunk (and with it all register indexing) are
-// uniform across the wave.
+// Work mapping: each workgroup pulls row tasks (up to 16 wave units of one
+// bra pair, longest first) from the class pair's queue; its four wavefronts take the task's
+// (unit, chunk) items in turn, so within a wave the bra pair and the chunk
+// (and with it all register indexing) are uniform. With LDS accumulators the
+// task's J[a,b] block and K rows of a and b are summed in LDS and added to
+// J/K once at the end of the task.
 //
 // Precision: with o.mixed a wave runs the FP32 instantiation when every active
 // quartet's bound Q_ab Q_cd max|D| is below tau, else the FP64 one; the vote
 // is uniform, so the wave never executes both.
+//
 // The read-only inputs are also passed as __restrict__ arguments and rebound
 // into b, w and o: with no possible aliasing store (the J/K atomics) the
 // compiler can turn wave-uniform loads (wave unit, bra pair, bra primitive
@@ -513,6 +554,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, 
   w.col_pair = col_pair;
   o.D = Dm;
   constexpr int NCH = PL::NCH, NR = PL::NR;
+  constexpr int NA = nc_(PL::la), NB = PL::NB;
   constexpr bool kLds = NR <= kLdsRootsMax;
   constexpr int TB = kLds ? rys_c::block_size(NR) : 1;
   __shared__ double t64[TB];
@@ -524,58 +566,136 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, 
       t64[t] = g64[t];
       t32[t] = g32[t];
     }
-    __syncthreads();
   }
+  const bool use_acc = w.acc_colsC > 0;
+  RowAcc acc{};
+  int nacc = 0;
+  if (use_acc) {  // row-task accumulators in s_acc, sized at launch
+    const int cC = w.acc_colsC, cD = w.acc_colsD;  // cD == 0: D columns share the C space
+    acc.on = true;
+    acc.colsC = cC;
+    acc.colsD = cD > 0 ? cD : cC;
+    acc.jab = 0;
+    acc.kac = NA * NB;
+    acc.kad = cD > 0 ? acc.kac + NA * cC : acc.kac;
+    acc.kbc = acc.kac + NA * cC + (cD > 0 ? NA * cD : 0);
+    acc.kbd = cD > 0 ? acc.kbc + NB * cC : acc.kbc;
+    nacc = NA * NB + (NA + NB) * (cC + cD);
+    for (int t = threadIdx.x; t < nacc; t += 256) s_acc[t] = 0.0;
+  }
+  __syncthreads();
   const double* tab64 = kLds ? t64 : g64;
   const float* tab32 = kLds ? t32 : g32;
   const int lane = threadIdx.x & 63;
   // launched with 256-thread blocks (4 waves); blockDim is not read because
-  // the compiler cannot prove it uniform, which would make the loop divergent
-  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-  const long long nwaves = (long long)gridDim.x * 4;
-  const long long total = w.nunits * NCH;
-  for (long long g = wave; g < total; g += nwaves) {  // uniform trip count per wave
-    const long long u = g / NCH;
-    const int chunk = (int)(g - u * NCH);
-    const WaveUnit un = w.units[u];
-    const int P = uni(un.P);
-    bool active = lane < un.ncol;
-    const int Qp = w.col_pair[un.col + (active ? lane : 0)];
-    double bound = 0.0;
-    if (active) {
-      bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
-      if (o.eps_density > 0.0) active = bound >= o.eps_density;
+  // the compiler cannot prove it uniform, which would make the loops divergent
+  const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  __shared__ int s_task;
+  for (;;) {  // persistent workgroups pull row tasks from the class pair's queue
+    if (threadIdx.x == 0) s_task = (int)atomicAdd(w.queue, 1u);
+    __syncthreads();
+    const int t = s_task;
+    if (t >= w.ntasks) break;  // uniform per workgroup; every workgroup gets here
+    const RowTask tk = w.tasks[t];
+    const int P = uni(w.units[tk.u0].P);
+    for (int item = wib; item < tk.nu * NCH; item += 4) {  // uniform per wave
+      const int ui = item / NCH;
+      const int chunk = item - ui * NCH;
+      const WaveUnit un = w.units[tk.u0 + ui];
+      bool active = lane < un.ncol;
+      const int Qp = w.col_pair[un.col + (active ? lane : 0)];
+      double bound = 0.0;
+      if (active) {
+        bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
+        if (o.eps_density > 0.0) active = bound >= o.eps_density;
+      }
+      if (!wave_any(active)) continue;  // the whole unit is density-screened
+      const bool lowp = o.mixed && wave_all(!active || bound < o.tau_mixed);
+      if (o.counters && chunk == 0) {
+        // statistics: one atomic per counter and wave
+        const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
+        unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
+        const unsigned long long kq =
+            active ? (unsigned long long)b.pp_count[P] * (unsigned long long)b.pp_count[Qp] : 0ull;
+        const int bin = (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp));
+        for (int k = 0; k < 8; ++k) {
+          const unsigned long long m = __ballot(active && bin == k);
+          if (m && lane == 0) atomicAdd(cnt + k, (unsigned long long)__popcll(m));
+        }
+        const unsigned long long ksum = (unsigned long long)wave_sum((double)kq);
+        const unsigned long long nact = (unsigned long long)__popcll(__ballot(active));
+        if (lane == 0) {
+          atomicAdd(cnt + 8, ksum);
+          if (lowp) {
+            atomicAdd(cnt + 9, nact);
+            atomicAdd(cnt + 10, ksum);
+          }
+        }
+      }
+      // inactive lanes (tail, screened) run with an empty primitive loop and no
+      // atomics so the wave stays converged for the cross-lane reductions
+      if (lowp)
+        dispatch_chunk<PL, 0, float>(chunk, b, o, P, Qp, active, tab32, acc);
+      else
+        dispatch_chunk<PL, 0, double>(chunk, b, o, P, Qp, active, tab64, acc);
     }
-    if (!wave_any(active)) continue;  // the whole unit is density-screened
-    const bool lowp = o.mixed && wave_all(!active || bound < o.tau_mixed);
-    if (o.counters && chunk == 0 && active) {
-      const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
-      unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
-      const unsigned long long kq = (unsigned long long)b.pp_count[P] * (unsigned long long)b.pp_count[Qp];
-      atomicAdd(cnt + (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp)), 1ull);
-      atomicAdd(cnt + 8, kq);
-      if (lowp) {
-        atomicAdd(cnt + 9, 1ull);
-        atomicAdd(cnt + 10, kq);
+    if (use_acc) {
+      __syncthreads();  // the task's contributions are in LDS
+      const int A = b.sp_A[P], B = b.sp_B[P];
+      const long long N = b.nao;
+      const long long aoA = b.shell_ao[A], aoB = b.shell_ao[B];
+      double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
+      double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
+      const int cC = w.acc_colsC, cD = w.acc_colsD;
+      const int* laoC = b.lao_ao + w.lao_offC;
+      const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD : w.lao_offC);
+      const int cDe = cD > 0 ? cD : cC;
+      for (int t2 = threadIdx.x; t2 < nacc; t2 += 256) {
+        const double v = s_acc[t2];
+        if (v == 0.0) continue;
+        s_acc[t2] = 0.0;
+        int r = t2;
+        if (r < NA * NB) {
+          atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], v);
+          continue;
+        }
+        r -= NA * NB;
+        if (r < NA * cC) {
+          atomicAdd(&Ko[(aoA + r / cC) * N + laoC[r % cC]], v);
+          continue;
+        }
+        r -= NA * cC;
+        if (cD > 0 && r < NA * cD) {
+          atomicAdd(&Ko[(aoA + r / cD) * N + laoD[r % cD]], v);
+          continue;
+        }
+        if (cD > 0) r -= NA * cD;
+        if (r < NB * cC) {
+          atomicAdd(&Ko[(aoB + r / cC) * N + laoC[r % cC]], v);
+          continue;
+        }
+        r -= NB * cC;
+        atomicAdd(&Ko[(aoB + r / cDe) * N + laoD[r % cDe]], v);
       }
     }
-    // inactive lanes (tail, screened) run with an empty primitive loop and no
-    // atomics so the wave stays converged for the cross-lane reductions
-    if (lowp)
-      dispatch_chunk<PL, 0, float>(chunk, b, o, P, Qp, active, tab32);
-    else
-      dispatch_chunk<PL, 0, double>(chunk, b, o, P, Qp, active, tab64);
+    __syncthreads();  // accumulators are zero again; s_task may be overwritten
   }
+}
+
+// Bytes of LDS accumulators for a class pair with ket columns (colsC, colsD).
+inline size_t row_acc_bytes(int la, int lb, int colsC, int colsD) {
+  const int NA = nc_(la), NB = nc_(lb);
+  return sizeof(double) * (size_t)(NA * NB + (NA + NB) * (colsC + colsD));
 }
 
 template <class PL>
 hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {
   const int block = 256;  // 4 waves
-  const long long waves = w.nunits * PL::NCH;
-  long long g = (waves + 3) / 4;
-  if (g > 16384) g = 16384;
+  long long g = w.ntasks;  // persistent: a few workgroups per CU, tasks from the queue
+  if (g > 2048) g = 2048;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(block), 0, s, b, w, o, b.ppf, w.units,
+  const size_t lds = w.acc_colsC > 0 ? row_acc_bytes(PL::la, PL::lb, w.acc_colsC, w.acc_colsD) : 0;
+  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(block), lds, s, b, w, o, b.ppf, w.units,
                      w.col_pair, b.sp_A, b.sp_B, b.pp_start, b.pp_count, b.shell_xyz, b.shell_ao, b.Q, b.Dsp, o.D);
   return hipGetLastError();
 }

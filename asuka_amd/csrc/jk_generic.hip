@@ -22,7 +22,7 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kNW = 16;  // components per lane per pass
+constexpr int kNW = 8;  // components per lane per pass
 
 // Cartesian components of l = 0..3 concatenated, offset l(l+1)(l+2)/6.
 __constant__ signed char c_lx[20] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0, 3, 2, 2, 1, 1, 1, 0, 0, 0, 0};
@@ -40,7 +40,9 @@ __device__ inline double ipow(double x, int k) {
   return r;
 }
 
-__global__ __launch_bounds__(kWave) void eri_generic_kernel(DevBasis b, ClassWork w, EriOut o) {
+// At most 128 VGPRs (4 waves per SIMD): the kernel is latency-bound, occupancy
+// matters more than the few spilled registers.
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void eri_generic_kernel(DevBasis b, ClassWork w, EriOut o) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
   double qmax = 0.0;

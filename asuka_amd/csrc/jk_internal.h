@@ -51,6 +51,8 @@ struct DevBasis {
   const int* pp_count;       // (nsp)
   const PrimPair* pp;
   const PrimPairF* ppf;      // same indexing as pp
+  const int* shell_lao;      // (nshell) first AO of the shell in the compact numbering of its l
+  const int* lao_ao;         // compact (l, index) -> AO, concatenated over l
   const double* Q;           // (nsp) Schwarz bounds
   const double* Dsp;         // (nsp) max |D| per shell-pair block
   int nao;
@@ -83,10 +85,22 @@ struct WaveUnit {
   int pad;
 };
 
+// A workgroup's share: consecutive wave units [u0, u0 + nu) of one bra row.
+struct RowTask {
+  int u0, nu;
+};
+
 struct ClassWork {
   int la, lb, lc, ld;
   const WaveUnit* units;  // class-specialised kernels (bra-uniform wavefronts)
   long long nunits;
+  const RowTask* tasks;
+  long long ntasks;
+  // LDS row accumulators of the ket class's columns: the number of AOs of
+  // angular momentum lc (acc_colsC) and ld (acc_colsD, 0 when lc == ld) and
+  // where their AO lists start in lao_ao; acc_colsC == 0 disables them.
+  int acc_colsC, acc_colsD, lao_offC, lao_offD;
+  unsigned int* queue;  // work-queue head (row tasks handed out in order), zero at launch
   const Segment* seg;
   int nseg;
   const int* row_pair;
@@ -156,3 +170,5 @@ __device__ inline bool decode_work(const ClassWork& w, long long k, int& P, int&
 
 // Launchers (defined in jk_generic.hip / jk_fast.hip).
 hipError_t launch_eri_generic(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s);
+// Wave-cooperative class templates (jk_coop.hip); *handled is false for classes it does not serve.
+hipError_t launch_eri_coop(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s, bool* handled);
