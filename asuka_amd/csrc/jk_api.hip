@@ -31,9 +31,14 @@ static thread_local std::string g_err;
 // default FP32 threshold on Q_ab Q_cd max|D| for builds with ASUKA_JK_MIXED
 static constexpr double kDefaultTauMixed = 1e-7;
 // Row tasks: at most this many wave units of one bra row per workgroup.
-static constexpr int kTaskUnits = 16;
-// LDS accumulators of one row task larger than this fall back to global atomics.
-static constexpr size_t kRowAccMaxBytes = 48 * 1024;
+static constexpr int kTaskUnits = 32;
+// LDS accumulators of one row task larger than this (or than the device allows
+// next to the root tables) fall back to global atomics.
+static constexpr size_t kRowAccMaxBytes = 72 * 1024;
+static constexpr size_t kRootTablesMaxBytes = 27 * 1024;
+// Accumulators up to this size run with 4-wave workgroups (16-unit tasks),
+// larger ones with 8-wave workgroups (32-unit tasks).
+static constexpr size_t kRowAccSmallBytes = 40 * 1024;
 // sqrt(2 pi^(5/2)), split evenly over the bra and ket primitive-pair factors
 static constexpr double kSqrt2Pi52 = 5.91496717279561287782;
 
@@ -60,6 +65,7 @@ struct ClassPairHost {
   long long unit_start, nunits;  // bra-uniform wavefront units
   long long task_start, ntasks;  // row tasks (workgroups)
   int acc_colsC, acc_colsD;      // LDS row accumulators (0: direct atomics)
+  int wpg;                       // wavefronts per workgroup (4 or 8)
   long long nquartets;  // Schwarz-screened quartets
   long long nprim_quartets;
   long long nprim_roots;
@@ -163,6 +169,7 @@ struct asuka_jk_ctx {
   WaveUnit* d_units = nullptr;
   RowTask* d_tasks = nullptr;
   int lao_off[ASUKA_LMAX + 2] = {0};
+  size_t lds_per_group = 64 * 1024;  // device limit of LDS per workgroup
   int nlao[ASUKA_LMAX + 1] = {0};
   double* d_Q = nullptr;
   double* d_Dsp = nullptr;
@@ -205,6 +212,11 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   auto* ctx = new asuka_jk_ctx();
   std::unique_ptr<asuka_jk_ctx> guard(ctx);
   HIP_CHECK(hipGetDevice(&ctx->device));
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
+      ctx->lds_per_group = (size_t)v;
+  }
   ctx->nshell = nshell;
   ctx->shell_l.assign(bd->shell_l, bd->shell_l + nshell);
   ctx->shell_ao.assign(bd->shell_ao_start, bd->shell_ao_start + nshell);
@@ -436,6 +448,17 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       cp.unit_start = (long long)units.size();
       cp.task_start = (long long)tasks.size();
       const int nroots = (cp.la + cp.lb + cp.lc + cp.ld) / 2 + 1;
+      int task_units = kTaskUnits;
+      {  // LDS row accumulators and workgroup size of this class pair
+        const int cC = ctx->nlao[cp.lc], cD = (cp.lc == cp.ld) ? 0 : ctx->nlao[cp.ld];
+        const size_t bytes = sizeof(double) * (size_t)(ncart_c(cp.la) * ncart_c(cp.lb) +
+                                                       (ncart_c(cp.la) + ncart_c(cp.lb)) * (cC + cD));
+        const bool fits = bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - kRootTablesMaxBytes);
+        cp.acc_colsC = fits ? cC : 0;
+        cp.acc_colsD = fits ? cD : 0;
+        cp.wpg = (fits && bytes > kRowAccSmallBytes) ? 8 : 4;
+        task_units = cp.wpg == 8 ? kTaskUnits : kTaskUnits / 2;
+      }
       long long padded = 0;
       for (const PairGroup& bg : groups[X]) {
         for (const PairGroup& kg : groups[Y]) {
@@ -463,7 +486,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
             {
               // the row's wave units, in row tasks of at most kTaskUnits
               const int nu = (int)((len + 63) / 64);
-              const int ntk = (nu + kTaskUnits - 1) / kTaskUnits;
+              const int ntk = (nu + task_units - 1) / task_units;
               for (int k = 0; k < ntk; ++k) {  // balanced task sizes
                 const int b0 = (int)((long long)nu * k / ntk), b1 = (int)((long long)nu * (k + 1) / ntk);
                 tasks.push_back(RowTask{(int)(units.size() - cp.unit_start) + b0, b1 - b0});
@@ -503,14 +526,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
         const long long cy = (long long)y.nu * ctx->pp_count[uy.P] * ctx->pp_count[col_all[uy.col]];
         return cx > cy;
       });
-      {
-        const int cC = ctx->nlao[cp.lc], cD = (cp.lc == cp.ld) ? 0 : ctx->nlao[cp.ld];
-        const size_t bytes = sizeof(double) * (size_t)(ncart_c(cp.la) * ncart_c(cp.lb) +
-                                                       (ncart_c(cp.la) + ncart_c(cp.lb)) * (cC + cD));
-        const bool fits = bytes <= kRowAccMaxBytes;
-        cp.acc_colsC = fits ? cC : 0;
-        cp.acc_colsD = fits ? cD : 0;
-      }
+
       if (cp.nseg == 0) continue;
       cp.nwork = padded;
       cp.nprim_roots = cp.nprim_quartets * nroots;
@@ -608,6 +624,7 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   w.lao_offC = ctx->lao_off[cp.lc];
   w.lao_offD = ctx->lao_off[cp.ld];
   w.queue = ctx->d_queue + index;
+  w.wpg = cp.wpg;
   return w;
 }
 

@@ -178,6 +178,12 @@ __device__ __forceinline__ float fmac(float a, float b, float c) { return __buil
 
 // Root tables of up to this many roots are copied to LDS by each workgroup.
 constexpr int kLdsRootsMax = 3;
+// Workgroups of up to 8 wavefronts share the root tables and the row
+// accumulators; the host picks 4 or 8 per class pair (ClassWork::wpg): 8 when
+// the accumulators are large (more waves per LDS byte), else 4 (less idle
+// time at the end of short row tasks).
+constexpr int kMaxWavesPerGroup = 8;
+constexpr int kGroupThreads = 64 * kMaxWavesPerGroup;
 
 // Workgroup accumulators of one row task (one bra pair P), in the dynamic
 // LDS array s_acc: the J[a,b] block and the K rows of the bra's AOs, K[a, :]
@@ -513,8 +519,8 @@ __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, con
   }
 }
 
-// Work mapping: each workgroup pulls row tasks (up to 16 wave units of one
-// bra pair, longest first) from the class pair's queue; its four wavefronts take the task's
+// Work mapping: each workgroup pulls row tasks (wave units of one bra pair,
+// longest first) from the class pair's queue; its eight wavefronts take the task's
 // (unit, chunk) items in turn, so within a wave the bra pair and the chunk
 // (and with it all register indexing) are uniform. With LDS accumulators the
 // task's J[a,b] block and K rows of a and b are summed in LDS and added to
@@ -529,7 +535,7 @@ __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, con
 // compiler can turn wave-uniform loads (wave unit, bra pair, bra primitive
 // pairs, D blocks of the bra) into scalar loads.
 template <class PL>
-__global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o,
+__global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o,
                                                        const PrimPairF* __restrict__ ppf,
                                                        const WaveUnit* __restrict__ units,
                                                        const int* __restrict__ col_pair,
@@ -557,12 +563,13 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, 
   constexpr int NA = nc_(PL::la), NB = PL::NB;
   constexpr bool kLds = NR <= kLdsRootsMax;
   constexpr int TB = kLds ? rys_c::block_size(NR) : 1;
+  const int wpg = w.wpg, nthr = 64 * w.wpg;  // the launch's block size (uniform)
   __shared__ double t64[TB];
   __shared__ float t32[TB];
   const double* g64 = rys_dev::rys_coef + rys_c::coef_off[NR];
   const float* g32 = rys_dev_f::rys_coef + rys_c::coef_off[NR];
   if constexpr (kLds) {
-    for (int t = threadIdx.x; t < TB; t += 256) {
+    for (int t = threadIdx.x; t < TB; t += nthr) {
       t64[t] = g64[t];
       t32[t] = g32[t];
     }
@@ -581,13 +588,13 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, 
     acc.kbc = acc.kac + NA * cC + (cD > 0 ? NA * cD : 0);
     acc.kbd = cD > 0 ? acc.kbc + NB * cC : acc.kbc;
     nacc = NA * NB + (NA + NB) * (cC + cD);
-    for (int t = threadIdx.x; t < nacc; t += 256) s_acc[t] = 0.0;
+    for (int t = threadIdx.x; t < nacc; t += nthr) s_acc[t] = 0.0;
   }
   __syncthreads();
   const double* tab64 = kLds ? t64 : g64;
   const float* tab32 = kLds ? t32 : g32;
   const int lane = threadIdx.x & 63;
-  // launched with 256-thread blocks (4 waves); blockDim is not read because
+  // launched with kGroupThreads-thread blocks; blockDim is not read because
   // the compiler cannot prove it uniform, which would make the loops divergent
   const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   __shared__ int s_task;
@@ -598,7 +605,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, 
     if (t >= w.ntasks) break;  // uniform per workgroup; every workgroup gets here
     const RowTask tk = w.tasks[t];
     const int P = uni(w.units[tk.u0].P);
-    for (int item = wib; item < tk.nu * NCH; item += 4) {  // uniform per wave
+    for (int item = wib; item < tk.nu * NCH; item += wpg) {  // uniform per wave
       const int ui = item / NCH;
       const int chunk = item - ui * NCH;
       const WaveUnit un = w.units[tk.u0 + ui];
@@ -650,7 +657,7 @@ __global__ __launch_bounds__(256) void eri_fast_kernel(DevBasis b, ClassWork w, 
       const int* laoC = b.lao_ao + w.lao_offC;
       const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD : w.lao_offC);
       const int cDe = cD > 0 ? cD : cC;
-      for (int t2 = threadIdx.x; t2 < nacc; t2 += 256) {
+      for (int t2 = threadIdx.x; t2 < nacc; t2 += nthr) {
         const double v = s_acc[t2];
         if (v == 0.0) continue;
         s_acc[t2] = 0.0;
@@ -690,12 +697,17 @@ inline size_t row_acc_bytes(int la, int lb, int colsC, int colsD) {
 
 template <class PL>
 hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {
-  const int block = 256;  // 4 waves
   long long g = w.ntasks;  // persistent: a few workgroups per CU, tasks from the queue
-  if (g > 2048) g = 2048;
+  const long long gmax = w.wpg == 8 ? 1024 : 2048;
+  if (g > gmax) g = gmax;
   if (g < 1) g = 1;
   const size_t lds = w.acc_colsC > 0 ? row_acc_bytes(PL::la, PL::lb, w.acc_colsC, w.acc_colsD) : 0;
-  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(block), lds, s, b, w, o, b.ppf, w.units,
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void*)eri_fast_kernel<PL>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(64 * w.wpg), lds, s, b, w, o, b.ppf, w.units,
                      w.col_pair, b.sp_A, b.sp_B, b.pp_start, b.pp_count, b.shell_xyz, b.shell_ao, b.Q, b.Dsp, o.D);
   return hipGetLastError();
 }

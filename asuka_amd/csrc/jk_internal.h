@@ -101,6 +101,7 @@ struct ClassWork {
   // where their AO lists start in lao_ao; acc_colsC == 0 disables them.
   int acc_colsC, acc_colsD, lao_offC, lao_offD;
   unsigned int* queue;  // work-queue head (row tasks handed out in order), zero at launch
+  int wpg;              // wavefronts per workgroup of the register kernels (4 or 8)
   const Segment* seg;
   int nseg;
   const int* row_pair;
