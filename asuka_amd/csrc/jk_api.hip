@@ -146,6 +146,9 @@ int dev_upload(Tp** dst, const std::vector<Tp>& src, std::vector<void*>& owned, 
 hipError_t launch_int1e(const asuka_jk_ctx* ctx, double* S, double* T, double* V, hipStream_t s);
 hipError_t launch_eri_deriv(const DevBasis& b, const ClassWork& w, const double* D, const double* WJ,
                             const double* WK, const int* shell_atom, double* grad, hipStream_t s);
+hipError_t launch_eri_deriv_fast(const DevBasis& b, const ClassWork& w, const double* D, const double* WJ,
+                                 const double* WK, const int* shell_atom, int natm, double* grad, hipStream_t s,
+                                 bool* handled);
 hipError_t launch_int1e_grad(const DevBasis& b, const int* pstart, const int* pnum, const double* pexp,
                              const double* pcoef, int natm, const double* axyz, const double* aZ,
                              const int* shell_atom, const double* Dh, const double* Ws, double* grad,
@@ -782,9 +785,16 @@ int asuka_jk_grad(asuka_jk_ctx* ctx, const double* D, const double* WJ, const do
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, grad, (long long)3 * ctx->natm, 0.0);
   HIP_CHECK(hipGetLastError());
   if (!WJ && !WK) return ASUKA_OK;
+  {  // work-queue heads of the lane-per-quartet kernels start at zero
+    const long long nq = (long long)std::max<size_t>(1, ctx->cps.size());
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ctx->d_queue, nq);
+    HIP_CHECK(hipGetLastError());
+  }
   for (size_t i = 0; i < ctx->cps.size(); ++i) {
     ClassWork w = work_of(ctx, ctx->cps[i], (int)i);
-    HIP_CHECK(launch_eri_deriv(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, grad, s));
+    bool handled = false;
+    HIP_CHECK(launch_eri_deriv_fast(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, ctx->natm, grad, s, &handled));
+    if (!handled) HIP_CHECK(launch_eri_deriv(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, grad, s));
   }
   return ASUKA_OK;
 }

@@ -26,7 +26,8 @@
 // translational invariance, dD = -(dA + dB + dC). The 2-D Rys tables are built
 // to la+lb+1 and lc+ld+1 with floor((L+1)/2)+1 roots.
 //
-// One wavefront per quartet; LDS holds the roots, the VRR tables of all roots,
+// One wavefront per quartet (the class pair's wave units, one ket at a time);
+// LDS holds the roots, the VRR tables of all roots,
 // the HRR-transferred 1-D tables of one root, and the adjoint tile of up to
 // kChunk components with their table offsets. Lanes own components; the nine
 // (centre A/B/C x xyz) sums are reduced across the wave at the end of the
@@ -102,9 +103,11 @@ __global__ __launch_bounds__(kWave) void eri_deriv_kernel(DevBasis b, ClassWork 
   const int sC = ld + 1, sB = (lc + 2) * sC, sA = (lb + 2) * sB;
   const long long N = b.nao;
 
-  for (long long k = blockIdx.x; k < w.nwork; k += gridDim.x) {
-    int P, Qp;
-    if (!decode_work(w, k, P, Qp)) continue;  // uniform across the wave
+  // item k: ket k % 64 of wave unit k / 64 (one quartet per wave at a time)
+  for (long long k = blockIdx.x; k < w.nunits * 64; k += gridDim.x) {
+    const WaveUnit un = w.units[k >> 6];
+    if ((int)(k & 63) >= un.ncol) continue;  // uniform across the wave
+    const int P = un.P, Qp = w.col_pair[un.col + (int)(k & 63)];
     const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
     const double* cA = b.shell_xyz + 3 * A;
     const double* cB = b.shell_xyz + 3 * B;
@@ -264,10 +267,10 @@ size_t eri_deriv_lds_bytes(int la, int lb, int lc, int ld) { return (size_t)lay_
 
 hipError_t launch_eri_deriv(const DevBasis& b, const ClassWork& w, const double* D, const double* WJ,
                             const double* WK, const int* shell_atom, double* grad, hipStream_t s) {
-  if (w.nwork <= 0) return hipSuccess;
+  if (w.nunits <= 0) return hipSuccess;
   GradArgs g{D, WJ, WK, shell_atom, grad};
   const size_t lds = eri_deriv_lds_bytes(w.la, w.lb, w.lc, w.ld);
-  long long grid = w.nwork < 16384 ? w.nwork : 16384;
+  long long grid = w.nunits * 64 < 32768 ? w.nunits * 64 : 32768;
   hipLaunchKernelGGL(eri_deriv_kernel, dim3((unsigned)grid), dim3(kWave), lds, s, b, w, g);
   return hipGetLastError();
 }
