@@ -1,334 +1,26 @@
-// Lane-per-quartet nuclear-derivative contraction for the low angular-momentum
-// classes (the bulk of the quartets), the backward pass of the J/K build.
-//
-// Same functional and adjoint tile as jk_grad.hip: per canonical quartet
-//   bar = fj (W_J[a,b] D[c,d] + [P!=Q] W_J[c,d] D[a,b])
-//       + fk (W_K[a,c] D[b,d] + [C!=D] W_K[a,d] D[b,c]
-//             + [A!=B] W_K[b,c] D[a,d] + [A!=B][C!=D] W_K[b,d] D[a,c]),
-// contracted with the centre derivatives of (ab|cd) per primitive quartet
-// (asuka/cueri/_eri_rys_cpu.pyx:4316-4656):
-//   d/dA_x (ab|cd) = 2 alpha (a+1_x b|cd) - a_x (a-1_x b|cd), likewise B, C;
-//   d/dD = -(d/dA + d/dB + d/dC) (translational invariance).
-//
-// Work mapping as in the forward register kernels: workgroups pull row tasks
-// (wave units of one bra pair) from the class pair's queue; a wave takes one
-// unit, its lanes one ket each, so the bra pair (atoms of A and B, bra
-// primitive pairs) is uniform across the wave. Per lane: the adjoint tile in
-// registers; per primitive quartet and root the 2-D tables G_x, G_y, G_z up to
-// (la+lb+1, lc+ld+1) in registers, and for every component the binomial
-// transfer to the 1-D factors it needs (all indices compile-time). The A and B
-// sums are reduced across the wave; all four centres' sums go to a per-
-// workgroup LDS gradient (natm x 3), added to the output once per workgroup.
-#include "jk_internal.h"
-#include "rys_eval.h"
+// Dispatch to the lane-per-quartet derivative kernels (jk_grad_fast_impl.h);
+// the class instantiations are spread over jk_grad_fast_p*.hip so they compile
+// in parallel.
+#include "jk_grad_fast_impl.h"
 
-namespace {
-
-constexpr int kWaves = 4;
-constexpr int kThreads = 64 * kWaves;
-constexpr int kLdsGradMaxAtoms = 512;  // LDS gradient accumulator up to this many atoms
-
-constexpr int ncg(int l) { return (l + 1) * (l + 2) / 2; }
-constexpr int cgx(int l, int i) {
-  int k = 0;
-  for (int x = l; x >= 0; --x)
-    for (int y = l - x; y >= 0; --y) {
-      if (k == i) return x;
-      ++k;
-    }
-  return 0;
-}
-constexpr int cgy(int l, int i) {
-  int k = 0;
-  for (int x = l; x >= 0; --x)
-    for (int y = l - x; y >= 0; --y) {
-      if (k == i) return y;
-      ++k;
-    }
-  return 0;
-}
-constexpr int cge(int l, int i, int ax) { return ax == 0 ? cgx(l, i) : ax == 1 ? cgy(l, i) : l - cgx(l, i) - cgy(l, i); }
-constexpr double binom_g(int n, int k) {
-  double r = 1.0;
-  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
-  return r;
-}
-
-__device__ __forceinline__ double wave_sum_g(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-__device__ __forceinline__ int uni_g(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-template <int LA, int LB, int LC, int LD>
-struct DPlan {
-  static constexpr int NA = ncg(LA), NB = ncg(LB), NC = ncg(LC), ND = ncg(LD);
-  static constexpr int NCOMP = NA * NB * NC * ND;
-  static constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
-  static constexpr int NMAX = LA + LB + 1, MMAX = LC + LD + 1;
-};
-
-// 1-D factor of (i j | k l) on one axis from the VRR table G[n][m] (binomial
-// transfer of the powers of j onto i and of l onto k); i..l compile-time.
-template <int NMAX, int MMAX, int I, int J, int K, int L>
-__device__ __forceinline__ double shift1(const double (&G)[NMAX + 1][MMAX + 1], const double* pab, const double* pcd) {
-  double out = 0.0;
-#pragma unroll
-  for (int n = 0; n <= J; ++n) {
-    double inner = 0.0;
-#pragma unroll
-    for (int m = 0; m <= L; ++m) inner += binom_g(L, m) * pcd[L - m] * G[I + n][K + m];
-    out += binom_g(J, n) * pab[J - n] * inner;
-  }
-  return out;
-}
-
-template <int LA, int LB, int LC, int LD, int CI>
-struct CompIdx {
-  static constexpr int NB = ncg(LB), NC = ncg(LC), ND = ncg(LD);
-  static constexpr int ia = CI / (NB * NC * ND), ib = (CI / (NC * ND)) % NB, ic = (CI / ND) % NC, id = CI % ND;
-  static constexpr int a(int ax) { return cge(LA, ia, ax); }
-  static constexpr int b(int ax) { return cge(LB, ib, ax); }
-  static constexpr int c(int ax) { return cge(LC, ic, ax); }
-  static constexpr int d(int ax) { return cge(LD, id, ax); }
-};
-
-template <int LA, int LB, int LC, int LD, int CI, int AX>
-__device__ __forceinline__ void axis_vals(const double (&G)[LA + LB + 2][LC + LD + 2], const double* pab, const double* pcd,
-                                          double a2, double b2, double c2, double& v0, double& vA, double& vB,
-                                          double& vC) {
-  using X = CompIdx<LA, LB, LC, LD, CI>;
-  constexpr int NM = LA + LB + 1, MM = LC + LD + 1;
-  constexpr int ea = X::a(AX), eb = X::b(AX), ec = X::c(AX), ed = X::d(AX);
-  v0 = shift1<NM, MM, ea, eb, ec, ed>(G, pab, pcd);
-  vA = a2 * shift1<NM, MM, ea + 1, eb, ec, ed>(G, pab, pcd);
-  if constexpr (ea > 0) vA -= ea * shift1<NM, MM, (ea > 0 ? ea - 1 : 0), eb, ec, ed>(G, pab, pcd);
-  vB = b2 * shift1<NM, MM, ea, eb + 1, ec, ed>(G, pab, pcd);
-  if constexpr (eb > 0) vB -= eb * shift1<NM, MM, ea, (eb > 0 ? eb - 1 : 0), ec, ed>(G, pab, pcd);
-  vC = c2 * shift1<NM, MM, ea, eb, ec + 1, ed>(G, pab, pcd);
-  if constexpr (ec > 0) vC -= ec * shift1<NM, MM, ea, eb, (ec > 0 ? ec - 1 : 0), ed>(G, pab, pcd);
-}
-
-template <int LA, int LB, int LC, int LD, int CI>
-__device__ __forceinline__ void comp_all(const double (&G)[3][LA + LB + 2][LC + LD + 2], const double (&pab)[3][LB + 2],
-                                         const double (&pcd)[3][LD + 1], const double* bar, double a2, double b2,
-                                         double c2, double (&gA)[3], double (&gB)[3], double (&gC)[3]) {
-  if constexpr (CI < ncg(LA) * ncg(LB) * ncg(LC) * ncg(LD)) {
-    double v0[3], vA[3], vB[3], vC[3];
-    axis_vals<LA, LB, LC, LD, CI, 0>(G[0], pab[0], pcd[0], a2, b2, c2, v0[0], vA[0], vB[0], vC[0]);
-    axis_vals<LA, LB, LC, LD, CI, 1>(G[1], pab[1], pcd[1], a2, b2, c2, v0[1], vA[1], vB[1], vC[1]);
-    axis_vals<LA, LB, LC, LD, CI, 2>(G[2], pab[2], pcd[2], a2, b2, c2, v0[2], vA[2], vB[2], vC[2]);
-    const double w = bar[CI];
-    const double yz = w * v0[1] * v0[2], xz = w * v0[0] * v0[2], xy = w * v0[0] * v0[1];
-    gA[0] = fma(vA[0], yz, gA[0]);
-    gA[1] = fma(vA[1], xz, gA[1]);
-    gA[2] = fma(vA[2], xy, gA[2]);
-    gB[0] = fma(vB[0], yz, gB[0]);
-    gB[1] = fma(vB[1], xz, gB[1]);
-    gB[2] = fma(vB[2], xy, gB[2]);
-    gC[0] = fma(vC[0], yz, gC[0]);
-    gC[1] = fma(vC[1], xz, gC[1]);
-    gC[2] = fma(vC[2], xy, gC[2]);
-    comp_all<LA, LB, LC, LD, CI + 1>(G, pab, pcd, bar, a2, b2, c2, gA, gB, gC);
-  }
-}
-
-struct GradFastArgs {
-  const double* D;
-  const double* WJ;  // may be null
-  const double* WK;  // may be null
-  const int* shell_atom;
-  double* grad;  // (natm, 3)
-  int natm;
-};
-
-template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(kThreads) void eri_deriv_fast_kernel(DevBasis b, ClassWork w, GradFastArgs g) {
-  using DP = DPlan<LA, LB, LC, LD>;
-  constexpr int NA = DP::NA, NB = DP::NB, NC = DP::NC, ND = DP::ND, NCOMP = DP::NCOMP;
-  constexpr int NR = DP::NR, NM = DP::NMAX, MM = DP::MMAX;
-  constexpr bool kLds = NR <= 3;
-  constexpr int TB = kLds ? rys_c::block_size(NR) : 1;
-  __shared__ double t64[TB];
-  __shared__ double sgrad[3 * kLdsGradMaxAtoms];
-  __shared__ int s_task;
-  const double* g64 = rys_dev::rys_coef + rys_c::coef_off[NR];
-  if constexpr (kLds)
-    for (int t = threadIdx.x; t < TB; t += kThreads) t64[t] = g64[t];
-  const bool lds_grad = g.natm <= kLdsGradMaxAtoms;
-  if (lds_grad)
-    for (int t = threadIdx.x; t < 3 * g.natm; t += kThreads) sgrad[t] = 0.0;
-  __syncthreads();
-  const double* tab = kLds ? t64 : g64;
-  const int lane = threadIdx.x & 63;
-  const int wib = uni_g((int)(threadIdx.x >> 6));
-  const long long N = b.nao;
-
-  auto add_grad = [&](int atom, int ax, double v) {
-    if (lds_grad)
-      atomicAdd(&sgrad[3 * atom + ax], v);
-    else
-      atomicAdd(&g.grad[3 * atom + ax], v);
-  };
-
-  for (;;) {
-    if (threadIdx.x == 0) s_task = (int)atomicAdd(w.queue, 1u);
-    __syncthreads();
-    const int t = s_task;
-    if (t >= w.ntasks) break;
-    const RowTask tk = w.tasks[t];
-    for (int item = wib; item < tk.nu; item += kWaves) {
-      const WaveUnit un = w.units[tk.u0 + item];
-      const int P = uni_g(un.P);
-      const bool active = lane < un.ncol;
-      const int Qp = w.col_pair[un.col + (active ? lane : 0)];
-      const int A = uni_g(b.sp_A[P]), B = uni_g(b.sp_B[P]), C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
-      const double* cA = b.shell_xyz + 3 * A;
-      const double* cB = b.shell_xyz + 3 * B;
-      const double* cC = b.shell_xyz + 3 * C;
-      const double* cD = b.shell_xyz + 3 * Dsh;
-      double pab[3][LB + 2], pcd[3][LD + 1];
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax) {
-        const double ab = cA[ax] - cB[ax], cd = cC[ax] - cD[ax];
-        pab[ax][0] = 1.0;
-        pcd[ax][0] = 1.0;
-#pragma unroll
-        for (int e = 1; e <= LB + 1; ++e) pab[ax][e] = pab[ax][e - 1] * ab;
-#pragma unroll
-        for (int e = 1; e <= LD; ++e) pcd[ax][e] = pcd[ax][e - 1] * cd;
-      }
-      // adjoint tile of this lane's quartet
-      const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
-      const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
-      const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
-      const double fk = bk ? 2.0 : 1.0;
-      double bar[NCOMP];
-#pragma unroll
-      for (int ci = 0; ci < NCOMP; ++ci) {
-        const int ia = ci / (NB * NC * ND), ib = (ci / (NC * ND)) % NB, ic = (ci / ND) % NC, id = ci % ND;
-        const long long a = aoA + ia, bb = aoB + ib, c = aoC + ic, d = aoD + id;
-        double v = 0.0;
-        if (active) {
-          if (g.WJ) v += fj * (g.WJ[a * N + bb] * g.D[c * N + d] + (bk ? g.WJ[c * N + d] * g.D[a * N + bb] : 0.0));
-          if (g.WK) {
-            double s = g.WK[a * N + c] * g.D[bb * N + d];
-            if (cd_neq) s += g.WK[a * N + d] * g.D[bb * N + c];
-            if (ab_neq) s += g.WK[bb * N + c] * g.D[a * N + d];
-            if (ab_neq && cd_neq) s += g.WK[bb * N + d] * g.D[a * N + c];
-            v += fk * s;
-          }
-        }
-        bar[ci] = v;
-      }
-      double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
-      const int ab0 = uni_g(b.pp_start[P]), nab = uni_g(b.pp_count[P]);
-      const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
-      for (int j = 0; j < ncd; ++j) {
-        const PrimPair Y = b.pp[cd0 + j];
-        const double q = Y.eta, c2 = 2.0 * Y.a_exp;
-        const double QC[3] = {Y.Px - cC[0], Y.Py - cC[1], Y.Pz - cC[2]};
-        for (int i = 0; i < nab; ++i) {
-          const PrimPair X = b.pp[ab0 + i];  // wave-uniform
-          const double p = X.eta, a2 = 2.0 * X.a_exp, b2 = 2.0 * X.b_exp;
-          const double spq = p + q, inv_pq = rcp_nr(spq);
-          const double dPQ[3] = {X.Px - Y.Px, X.Py - Y.Py, X.Pz - Y.Pz};
-          const double T = p * q * inv_pq * (dPQ[0] * dPQ[0] + dPQ[1] * dPQ[1] + dPQ[2] * dPQ[2]);
-          const double base = 34.98683665524972497 * X.inv_eta * Y.inv_eta * rsq_nr(spq) * X.cK * Y.cK;
-          const double PA[3] = {X.Px - cA[0], X.Py - cA[1], X.Pz - cA[2]};
-          double rx[NR], rwt[NR];
-          rys_roots_blk<NR, double>(T, tab, rx, rwt);
-#pragma unroll
-          for (int u = 0; u < NR; ++u) {
-            const double x = rx[u];
-            const double B0 = 0.5 * x * inv_pq;
-            const double B1 = 0.5 * (1.0 - x) * X.inv_eta + B0;
-            const double B1p = 0.5 * (1.0 - x) * Y.inv_eta + B0;
-            const double xq = q * inv_pq * x, xp = p * inv_pq * x;
-            double G[3][NM + 1][MM + 1];
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-              const double Cc = PA[ax] - xq * dPQ[ax], Cp = QC[ax] + xp * dPQ[ax];
-              G[ax][0][0] = (ax == 2) ? rwt[u] * base : 1.0;
-              G[ax][1][0] = Cc * G[ax][0][0];
-#pragma unroll
-              for (int a = 2; a <= NM; ++a) G[ax][a][0] = B1 * (a - 1) * G[ax][a - 2][0] + Cc * G[ax][a - 1][0];
-#pragma unroll
-              for (int a = 0; a <= NM; ++a) {
-                G[ax][a][1] = Cp * G[ax][a][0] + (a > 0 ? a * B0 * G[ax][a > 0 ? a - 1 : 0][0] : 0.0);
-#pragma unroll
-                for (int c = 2; c <= MM; ++c)
-                  G[ax][a][c] = B1p * (c - 1) * G[ax][a][c - 2] + Cp * G[ax][a][c - 1] +
-                                (a > 0 ? a * B0 * G[ax][a > 0 ? a - 1 : 0][c - 1] : 0.0);
-              }
-            }
-            comp_all<LA, LB, LC, LD, 0>(G, pab, pcd, bar, a2, b2, c2, gA, gB, gC);
-          }
-        }
-      }
-      // centres A and B are the same on every lane: reduce across the wave
-      const int atA = g.shell_atom[A], atB = g.shell_atom[B];
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax) {
-        const double sa = wave_sum_g(gA[ax]), sb = wave_sum_g(gB[ax]);
-        if (lane == 0) {
-          add_grad(atA, ax, sa);
-          add_grad(atB, ax, sb);
-        }
-      }
-      if (active) {
-        const int atC = g.shell_atom[C], atD = g.shell_atom[Dsh];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-          add_grad(atC, ax, gC[ax]);
-          add_grad(atD, ax, -(gA[ax] + gB[ax] + gC[ax]));
-        }
-      }
-    }
-    __syncthreads();  // s_task may be overwritten
-  }
-  if (lds_grad) {
-    for (int t = threadIdx.x; t < 3 * g.natm; t += kThreads) {
-      const double v = sgrad[t];
-      if (v != 0.0) atomicAdd(&g.grad[t], v);
-    }
-  }
-}
-
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_deriv_fast(const DevBasis& b, const ClassWork& w, const GradFastArgs& g, hipStream_t s) {
-  long long grid = w.ntasks < 2048 ? w.ntasks : 2048;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((eri_deriv_fast_kernel<LA, LB, LC, LD>), dim3((unsigned)grid), dim3(kThreads), 0, s, b, w, g);
-  return hipGetLastError();
-}
-
-using DerivLauncher = hipError_t (*)(const DevBasis&, const ClassWork&, const GradFastArgs&, hipStream_t);
-
-#define ASUKA_DERIV_FAST_LIST(X) \
-  X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(1, 1, 1, 0) X(2, 0, 0, 0) X(2, 0, 1, 0) X(2, 1, 0, 0) \
-  X(2, 2, 0, 0) X(3, 0, 0, 0) X(3, 0, 1, 0)
-
-DerivLauncher find_deriv_fast(int la, int lb, int lc, int ld) {
-#define ASUKA_DERIV_CASE(a, b_, c, d) \
-  if (la == a && lb == b_ && lc == c && ld == d) return &launch_deriv_fast<a, b_, c, d>;
-  ASUKA_DERIV_FAST_LIST(ASUKA_DERIV_CASE)
-#undef ASUKA_DERIV_CASE
-  return nullptr;
-}
-
-}  // namespace
+asuka_gradf::DerivLauncher find_gradf_p0(int la, int lb, int lc, int ld);
+asuka_gradf::DerivLauncher find_gradf_p1(int la, int lb, int lc, int ld);
+asuka_gradf::DerivLauncher find_gradf_p2(int la, int lb, int lc, int ld);
+asuka_gradf::DerivLauncher find_gradf_p3(int la, int lb, int lc, int ld);
 
 hipError_t launch_eri_deriv_fast(const DevBasis& b, const ClassWork& w, const double* D, const double* WJ,
                                  const double* WK, const int* shell_atom, int natm, double* grad, hipStream_t s,
                                  bool* handled) {
   *handled = false;
   if (!w.units || !w.tasks) return hipSuccess;
-  DerivLauncher f = find_deriv_fast(w.la, w.lb, w.lc, w.ld);
+  asuka_gradf::DerivLauncher f = nullptr;
+  asuka_gradf::DerivLauncher (*finders[])(int, int, int, int) = {find_gradf_p0, find_gradf_p1, find_gradf_p2,
+                                                                  find_gradf_p3};
+  for (auto fn : finders)
+    if (!f) f = fn(w.la, w.lb, w.lc, w.ld);
   if (!f) return hipSuccess;
   *handled = true;
   if (w.ntasks <= 0) return hipSuccess;
-  GradFastArgs g{D, WJ, WK, shell_atom, grad, natm};
+  asuka_gradf::GradFastArgs g{D, WJ, WK, shell_atom, grad, natm};
   return f(b, w, g, s);
 }
