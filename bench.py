@@ -36,7 +36,97 @@ CONFIGS = {
             workload="(H2O)8 cc-pVTZ cart, one direct J/K build"),
     5: dict(n=64, seed=5, basis="cc-pVDZ", nocc=320, cseed=1005, eps=1e-12,
             workload="(H2O)64 cc-pVDZ cart, one direct Fock build"),
+    4: dict(n=16, seed=4, basis="cc-pVDZ", eps=1e-12,
+            workload="(H2O)16 cc-pVDZ cart, converged RHF D: forward J/K(D,R) + backward dE/dR"),
 }
+
+
+def run_gradient(args, cfg) -> int:
+    """Gradient workload: one step = forward J/K through the autograd Function
+    at the converged RHF density, E_2e = 1/2 Tr(D J) - 1/4 Tr(D K), and the
+    hand-written backward to dE_2e/dR (plus dE/dD), FP64."""
+    import torch
+
+    from asuka_amd.frontend import run_hf_df
+    from asuka_amd.hf.grad import direct_jk_autograd, rhf_nuc_grad
+    from asuka_amd.molecule import water_cluster
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    mol = water_cluster(cfg["n"], cfg["seed"], cfg["basis"])
+    t0 = time.perf_counter()
+    res = run_hf_df(mol, two_e_backend="direct", eps_schwarz=cfg["eps"], conv_tol=1e-10)
+    t_scf = time.perf_counter() - t0
+    ctx = res.direct_jk_ctx
+    log(f"SCF: nao={ctx.nao} E={res.scf.e_tot:.10f} converged={res.scf.converged} in {t_scf:.1f}s")
+    D0 = res.scf.dm.detach().clone()
+    R0 = torch.tensor(mol.coords_bohr, device=dev)
+
+    def step():
+        D = D0.clone().requires_grad_(True)
+        R = R0.clone().requires_grad_(True)
+        J, K = direct_jk_autograd(ctx, D, R)
+        E = 0.5 * (D * J).sum() - 0.25 * (D * K).sum()
+        E.backward()
+        return R.grad
+
+    for _ in range(max(0, args.warmup)):
+        step()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g2 = step()
+    torch.cuda.synchronize(dev)
+    s_per = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    g = rhf_nuc_grad(res)
+    torch.cuda.synchronize(dev)
+    t_full = time.perf_counter() - t0
+    fd = None
+    if args.fd:
+        # central differences (step 1e-4 Bohr) of the converged SCF energy on 6
+        # coordinates drawn with default_rng(seed).choice(3 natm, 6)
+        h = 1e-4
+        picks = np.random.default_rng(cfg["seed"]).choice(3 * mol.natm, 6, replace=False)
+        R = np.asarray(mol.coords_bohr, dtype=np.float64)
+        worst = 0.0
+        for k in picks:
+            a, x = divmod(int(k), 3)
+            e = []
+            for sgn in (-1, 1):
+                Rk = R.copy()
+                Rk[a, x] += sgn * h
+                r = run_hf_df(mol.with_coords(Rk), two_e_backend="direct", eps_schwarz=cfg["eps"], conv_tol=1e-12,
+                              conv_tol_dm=1e-9, dm0=res.scf.dm)
+                e.append(r.scf.e_tot)
+            fdv = (e[1] - e[0]) / (2 * h)
+            worst = max(worst, abs(fdv - float(g[a, x])))
+            log(f"FD atom {a} axis {x}: analytic {g[a, x]:.8f} FD {fdv:.8f}")
+        fd = {"coordinates": [int(k) for k in picks], "step_bohr": h, "max_abs_diff_eh_per_bohr": worst}
+    out = {
+        "metric": "gradient s/eval (forward J/K + backward dE/dR, FP64)",
+        "value": 1.0 / s_per,
+        "unit": "gradient_evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s_per * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded water cluster geometry), converged RHF density",
+        "config": {"workload": cfg["workload"], "nao": ctx.nao, "basis": cfg["basis"], "n_water": cfg["n"],
+                   "geometry_seed": cfg["seed"], "eps_schwarz": cfg["eps"]},
+        "s_per_eval": s_per,
+        "full_rhf_gradient_s": t_full,
+        "max_abs_translational_sum": float(abs(g.sum(axis=0)).max()),
+        "e_tot": res.scf.e_tot,
+        "grad_2e_norm": float(g2.norm()),
+        "fd_check": fd,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def log(*a):
@@ -85,6 +175,7 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fd", action="store_true", help="gradient workload: central-difference check on 6 coordinates")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is design-only "
@@ -99,6 +190,8 @@ def main() -> int:
     from asuka_amd.molecule import water_cluster
 
     cfg = CONFIGS[args.config]
+    if args.config == 4:
+        return run_gradient(args, cfg)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     mol = water_cluster(cfg["n"], cfg["seed"], cfg["basis"])
