@@ -76,6 +76,8 @@ SYMBOLS = [
     ("asuka_jk_last_counts", ctypes.c_int, [_vp, _i64p, ctypes.c_int, _vp]),
     ("asuka_jk_eri_tiles", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_int1e", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("asuka_jk_build_batched", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
     ("asuka_jk_grad", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("asuka_int1e_grad", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("asuka_rys_roots", ctypes.c_int, [ctypes.c_int, ctypes.c_double, _f64p, _f64p]),

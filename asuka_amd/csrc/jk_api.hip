@@ -728,6 +728,20 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   return ASUKA_OK;
 }
 
+int asuka_jk_build_batched(int nmol, asuka_jk_ctx* const* ctxs, const double* const* D, double* const* J,
+                           double* const* K, int flags, void* stream) {
+  if (nmol < 0) return fail(ASUKA_E_INVALID, "nmol must be >= 0");
+  if (nmol > 0 && (!ctxs || !D)) return fail(ASUKA_E_INVALID, "ctxs and D must be non-null");
+  for (int m = 0; m < nmol; ++m)
+    if (!ctxs[m] || !D[m]) return fail(ASUKA_E_INVALID, "every ctxs[m] and D[m] must be non-null");
+  for (int m = 0; m < nmol; ++m) {
+    const int rc = asuka_jk_build(ctxs[m], D[m], J ? J[m] : nullptr, K ? K[m] : nullptr, flags & ~ASUKA_JK_TIME,
+                                  stream);
+    if (rc) return rc;
+  }
+  return ASUKA_OK;
+}
+
 int asuka_jk_last_timing(asuka_jk_ctx* ctx, double* ms, int64_t* nlaunch) {
   if (!ctx || !ms) return fail(ASUKA_E_INVALID, "ctx and ms must be non-null");
   if (!ctx->ev_start || !ctx->ev_stop) return fail(ASUKA_E_INVALID, "no timed build recorded (use ASUKA_JK_TIME)");

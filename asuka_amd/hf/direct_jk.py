@@ -266,6 +266,30 @@ def direct_fock_rhf(
     return F
 
 
+def direct_JK_batched(ctxs, Ds, *, want_J: bool = True, want_K: bool = True, mixed_precision: bool | None = None,
+                      out=None):
+    """J and K of several independent molecules in one call (one context per
+    molecule, issued back to back on the current stream with no host sync, so
+    a HIP graph can capture the whole batch). Returns a list of (J, K)."""
+    torch = _torch()
+    n = len(ctxs)
+    if len(Ds) != n:
+        raise ValueError("Ds must have one density per context")
+    Dd = [_as_device_matrix(torch, D, c.nao, "D", c.device) for c, D in zip(ctxs, Ds)]
+    if out is None:
+        out = [(torch.empty((c.nao, c.nao), dtype=torch.float64, device=c.device) if want_J else None,
+                torch.empty((c.nao, c.nao), dtype=torch.float64, device=c.device) if want_K else None) for c in ctxs]
+    arr = ctypes.c_void_p * max(1, n)
+    hs = arr(*[ctypes.c_void_p(c.handle) for c in ctxs])
+    dp = arr(*[ctypes.c_void_p(_lib.ptr(d)) for d in Dd])
+    jp = arr(*[ctypes.c_void_p(_lib.ptr(o[0])) for o in out])
+    kp = arr(*[ctypes.c_void_p(_lib.ptr(o[1])) for o in out])
+    flags = (_lib.WANT_J if want_J else 0) | (_lib.WANT_K if want_K else 0) | _flags(mixed_precision)
+    _lib.check(_lib.lib().asuka_jk_build_batched(n, hs, dp, jp, kp, flags,
+                                                 ctypes.c_void_p(_lib.stream_handle(ctxs[0].device if n else None))))
+    return out
+
+
 def eri_tiles(ctx: DirectJKContext, spab, spcd):
     """Contracted ERI tiles for explicit (spAB, spCD) quartets (test hook).
     Returns a list of (nAB, nCD) numpy arrays in shell-pair orientation."""
@@ -292,4 +316,5 @@ def eri_tiles(ctx: DirectJKContext, spab, spcd):
     return [host[o:o + s].reshape(shp) for o, s, shp in zip(off, sizes, shapes)]
 
 
-__all__ = ["DirectJKContext", "direct_JK", "direct_fock_rhf", "eri_tiles", "make_direct_jk_context"]
+__all__ = ["DirectJKContext", "direct_JK", "direct_JK_batched", "direct_fock_rhf", "eri_tiles",
+           "make_direct_jk_context"]

@@ -142,6 +142,15 @@ int asuka_jk_eri_tiles(asuka_jk_ctx* ctx, int64_t nq, const int32_t* spab, const
  * (asuka/integrals/int1e_cart.py:1423-1434). */
 int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream);
 
+/* J/K of nmol independent molecules (one context each) in one call on one
+ * stream: ctxs[m], D[m], J[m], K[m] as for asuka_jk_build (J or K may be NULL
+ * arrays, or hold NULL entries). The builds are issued back to back without
+ * host synchronisation, so the whole batch can be captured in one HIP graph.
+ * The replacement of the per-molecule loop over direct_JK calls (the batched
+ * multi-molecule workload); the reference has no batched entry point. */
+int asuka_jk_build_batched(int nmol, asuka_jk_ctx* const* ctxs, const double* const* D, double* const* J,
+                           double* const* K, int flags, void* stream);
+
 /* Backward pass of the J/K build with respect to the nuclear coordinates:
  * grad (natm x 3, device, overwritten) = dL/dR for
  *   L = Tr(WJ J(D)) + Tr(WK K(D))
