@@ -69,6 +69,7 @@ SYMBOLS = [
     ("asuka_jk_ctx_destroy", ctypes.c_int, [_vp]),
     ("asuka_jk_ctx_info", ctypes.c_int, [_vp, _i64p]),
     ("asuka_jk_build", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    ("asuka_jk_build_multi", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_fock_build", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_jk_shell_pairs", ctypes.c_int, [_vp, _i32p, _i32p, _f64p]),
     ("asuka_jk_class_stats", ctypes.c_int, [_vp, _i64p, ctypes.c_int]),

@@ -102,9 +102,10 @@ __global__ void k_symmetrize(double* X, int n) {
   X[(long long)j * n + i] = m;
 }
 
-// max |D| over each shell-pair block, both orientations (asuka/hf/direct_jk.py:1484-1527).
-__global__ void k_dsp(const double* D, const int* sp_A, const int* sp_B, const int* shell_ao, const int* shell_l, int nsp,
-                      int nao, double* Dsp) {
+// max |D| over each shell-pair block, both orientations (asuka/hf/direct_jk.py:1484-1527);
+// with a second density D2 (two-density builds) the max over both.
+__global__ void k_dsp(const double* D, const double* D2, const int* sp_A, const int* sp_B, const int* shell_ao,
+                      const int* shell_l, int nsp, int nao, double* Dsp) {
   const int sp = blockIdx.x * blockDim.x + threadIdx.x;
   if (sp >= nsp) return;
   const int A = sp_A[sp], B = sp_B[sp];
@@ -115,6 +116,10 @@ __global__ void k_dsp(const double* D, const int* sp_A, const int* sp_B, const i
     for (int j = 0; j < nB; ++j) {
       m = fmax(m, fabs(D[(long long)(a0 + i) * nao + b0 + j]));
       if (A != B) m = fmax(m, fabs(D[(long long)(b0 + j) * nao + a0 + i]));
+      if (D2) {
+        m = fmax(m, fabs(D2[(long long)(a0 + i) * nao + b0 + j]));
+        if (A != B) m = fmax(m, fabs(D2[(long long)(b0 + j) * nao + a0 + i]));
+      }
     }
   Dsp[sp] = m;
 }
@@ -645,6 +650,11 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
   long long n = 0;
   for (size_t i = 0; i < ctx->cps.size(); ++i) {
     ClassWork w = work_of(ctx, ctx->cps[i], (int)i);
+    if (o.D2 && w.acc_colsC > 0) {  // two accumulator sets must fit the workgroup's LDS
+      const size_t nA = ncart_c(w.la), nB = ncart_c(w.lb);
+      const size_t bytes = 2 * sizeof(double) * (nA * nB + (nA + nB) * (size_t)(w.acc_colsC + w.acc_colsD));
+      if (bytes > ctx->lds_per_group - kRootTablesMaxBytes) w.acc_colsC = w.acc_colsD = 0;
+    }
     bool handled = false;
     HIP_CHECK(launch_eri_fast(ctx->dev, w, o, s, &handled));
     if (!handled) HIP_CHECK(launch_eri_coop(ctx->dev, w, o, s, &handled));
@@ -661,25 +671,28 @@ static inline unsigned grid_for(long long n, int block) {
   return (unsigned)std::max<long long>(1, std::min<long long>(g, 65535));
 }
 
-extern "C" {
-
-int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream) {
-  if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
-  hipStream_t s = (hipStream_t)stream;
-  const bool wantJ = (flags & ASUKA_JK_WANT_J) && J;
-  const bool wantK = (flags & ASUKA_JK_WANT_K) && K;
-  if (!wantJ && !wantK) return ASUKA_OK;
+// J/K of one density (D2 == nullptr) or of two densities from one evaluation
+// of each quartet; the outputs are zeroed first and symmetrised at the end.
+static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, double* J, double* K, double* J2,
+                         double* K2, bool wantJ, bool wantK, int flags, hipStream_t s) {
   const long long nn = (long long)ctx->nao * ctx->nao;
-  if (wantJ) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, J, nn, 0.0);
-  if (wantK) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, K, nn, 0.0);
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, ctx->dev.sp_A, ctx->dev.sp_B,
+  double* outs[4] = {wantJ ? J : nullptr, wantK ? K : nullptr, (D2 && wantJ) ? J2 : nullptr,
+                     (D2 && wantK) ? K2 : nullptr};
+  for (double* X : outs)
+    if (X) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, X, nn, 0.0);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, D2, ctx->dev.sp_A, ctx->dev.sp_B,
                      ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_JK;
   o.D = D;
-  o.J = wantJ ? J : nullptr;
-  o.K = wantK ? K : nullptr;
+  o.J = outs[0];
+  o.K = outs[1];
+  if (D2) {
+    o.D2 = D2;
+    o.J2 = outs[2];
+    o.K2 = outs[3];
+  }
   o.eps_density = ctx->eps_density;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
   o.tau_mixed = ctx->tau_mixed;
@@ -692,10 +705,35 @@ int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int
   if (rc) return rc;
   const int n = ctx->nao;
   dim3 blk(16, 16), grd((n + 15) / 16, (n + 15) / 16);
-  if (wantJ) hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, s, J, n);
-  if (wantK) hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, s, K, n);
+  for (double* X : outs)
+    if (X) hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, s, X, n);
   HIP_CHECK(hipGetLastError());
   return ASUKA_OK;
+}
+
+extern "C" {
+
+int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream) {
+  if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  const bool wantJ = (flags & ASUKA_JK_WANT_J) && J;
+  const bool wantK = (flags & ASUKA_JK_WANT_K) && K;
+  if (!wantJ && !wantK) return ASUKA_OK;
+  return jk_build_impl(ctx, D, nullptr, J, K, nullptr, nullptr, wantJ, wantK, flags, (hipStream_t)stream);
+}
+
+int asuka_jk_build_multi(asuka_jk_ctx* ctx, const double* Da, const double* Db, double* Ja, double* Ka, double* Jb,
+                         double* Kb, int flags, void* stream) {
+  if (!ctx || !Da || !Db) return fail(ASUKA_E_INVALID, "ctx, Da and Db must be non-null");
+  const bool wantJ = (flags & ASUKA_JK_WANT_J) != 0;
+  const bool wantK = (flags & ASUKA_JK_WANT_K) != 0;
+  if (wantJ && (!Ja || !Jb)) return fail(ASUKA_E_INVALID, "Ja and Jb must be non-null with ASUKA_JK_WANT_J");
+  if (wantK && (!Ka || !Kb)) return fail(ASUKA_E_INVALID, "Ka and Kb must be non-null with ASUKA_JK_WANT_K");
+  if (!wantJ && !wantK) return ASUKA_OK;
+  const double* outs[4] = {Ja, Ka, Jb, Kb};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < i; ++j)
+      if (outs[i] && outs[i] == outs[j]) return fail(ASUKA_E_INVALID, "Ja, Ka, Jb and Kb must be distinct buffers");
+  return jk_build_impl(ctx, Da, Db, Ja, Ka, Jb, Kb, wantJ, wantK, flags, (hipStream_t)stream);
 }
 
 int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, double* F, int flags, void* stream) {
@@ -704,7 +742,7 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   hipStream_t s = (hipStream_t)stream;
   const long long nn = (long long)ctx->nao * ctx->nao;
   if (F != hcore) hipLaunchKernelGGL(k_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, F, hcore, nn);
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, ctx->dev.sp_A, ctx->dev.sp_B,
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev.sp_A, ctx->dev.sp_B,
                      ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};

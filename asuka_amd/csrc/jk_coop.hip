@@ -76,8 +76,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
   double* Dac = Dad + NA * ND;
   const int lane = threadIdx.x;
   const long long N = b.nao;
-  double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
-  double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
   constexpr int oA = LA * (LA + 1) * (LA + 2) / 6, oB = LB * (LB + 1) * (LB + 2) / 6;
   constexpr int oC = LC * (LC + 1) * (LC + 2) / 6, oD = LD * (LD + 1) * (LD + 2) / 6;
 
@@ -209,66 +207,83 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
       }
       __syncthreads();  // tile complete
       // ---- digestion: each lane owns block elements; no atomics inside the wave ----
+      // (two-density builds digest the tile a second time with D2 into J2, K2)
       const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
       const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
       const double fk = (bk ? 2.0 : 1.0) * (o.mode == MODE_FOCK ? -0.5 : 1.0);
-      for (int e = lane; e < K::NOUT; e += kWave) {
-        int r = e;
-        if (r < NAB) {  // J[a,b] += fj sum_cd v D[c,d]
-          if (!Jo) continue;
-          double s = 0.0;
-          for (int icd = 0; icd < NCD; ++icd) s += tile[r * NCD + icd] * Dcd[icd];
-          atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], fj * s);
-          continue;
+      const int nden = o.D2 ? 2 : 1;
+      for (int di = 0; di < nden; ++di) {
+        double* Jo = di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
+        double* Ko = di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+        if (di) {  // the density blocks of D2 replace those of D
+          __syncthreads();
+          const double* D2 = o.D2;
+          for (int e = lane; e < NCD; e += kWave) Dcd[e] = D2[(aoC + e / ND) * N + aoD + e % ND];
+          for (int e = lane; e < NAB; e += kWave) Dab[e] = D2[(aoA + e / NB) * N + aoB + e % NB];
+          for (int e = lane; e < NB * ND; e += kWave) Dbd[e] = D2[(aoB + e / ND) * N + aoD + e % ND];
+          for (int e = lane; e < NB * NC; e += kWave) Dbc[e] = D2[(aoB + e / NC) * N + aoC + e % NC];
+          for (int e = lane; e < NA * ND; e += kWave) Dad[e] = D2[(aoA + e / ND) * N + aoD + e % ND];
+          for (int e = lane; e < NA * NC; e += kWave) Dac[e] = D2[(aoA + e / NC) * N + aoC + e % NC];
+          __syncthreads();
         }
-        r -= NAB;
-        if (r < NCD) {  // J[c,d] += fj sum_ab v D[a,b]
-          if (!Jo || !bk) continue;
-          double s = 0.0;
-          for (int iab = 0; iab < NAB; ++iab) s += tile[iab * NCD + r] * Dab[iab];
-          atomicAdd(&Jo[(aoC + r / ND) * N + aoD + r % ND], fj * s);
-          continue;
-        }
-        r -= NCD;
-        if (!Ko) continue;
-        if (r < NA * NC) {  // K[a,c] += sum_{b,d} v D[b,d]
-          const int ia = r / NC, ic = r - ia * NC;
-          double s = 0.0;
-          for (int ib = 0; ib < NB; ++ib)
-            for (int id = 0; id < ND; ++id) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dbd[ib * ND + id];
-          atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], fk * s);
-          continue;
-        }
-        r -= NA * NC;
-        if (r < NA * ND) {  // K[a,d] += sum_{b,c} v D[b,c]
+        for (int e = lane; e < K::NOUT; e += kWave) {
+          int r = e;
+          if (r < NAB) {  // J[a,b] += fj sum_cd v D[c,d]
+            if (!Jo) continue;
+            double s = 0.0;
+            for (int icd = 0; icd < NCD; ++icd) s += tile[r * NCD + icd] * Dcd[icd];
+            atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], fj * s);
+            continue;
+          }
+          r -= NAB;
+          if (r < NCD) {  // J[c,d] += fj sum_ab v D[a,b]
+            if (!Jo || !bk) continue;
+            double s = 0.0;
+            for (int iab = 0; iab < NAB; ++iab) s += tile[iab * NCD + r] * Dab[iab];
+            atomicAdd(&Jo[(aoC + r / ND) * N + aoD + r % ND], fj * s);
+            continue;
+          }
+          r -= NCD;
+          if (!Ko) continue;
+          if (r < NA * NC) {  // K[a,c] += sum_{b,d} v D[b,d]
+            const int ia = r / NC, ic = r - ia * NC;
+            double s = 0.0;
+            for (int ib = 0; ib < NB; ++ib)
+              for (int id = 0; id < ND; ++id) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dbd[ib * ND + id];
+            atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], fk * s);
+            continue;
+          }
+          r -= NA * NC;
+          if (r < NA * ND) {  // K[a,d] += sum_{b,c} v D[b,c]
+            if (!cd_neq) continue;
+            const int ia = r / ND, id = r - ia * ND;
+            double s = 0.0;
+            for (int ib = 0; ib < NB; ++ib)
+              for (int ic = 0; ic < NC; ++ic) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dbc[ib * NC + ic];
+            atomicAdd(&Ko[(aoA + ia) * N + aoD + id], fk * s);
+            continue;
+          }
+          r -= NA * ND;
+          if (!ab_neq) continue;
+          if (r < NB * NC) {  // K[b,c] += sum_{a,d} v D[a,d]
+            const int ib = r / NC, ic = r - ib * NC;
+            double s = 0.0;
+            for (int ia = 0; ia < NA; ++ia)
+              for (int id = 0; id < ND; ++id) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dad[ia * ND + id];
+            atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], fk * s);
+            continue;
+          }
+          r -= NB * NC;
           if (!cd_neq) continue;
-          const int ia = r / ND, id = r - ia * ND;
-          double s = 0.0;
-          for (int ib = 0; ib < NB; ++ib)
-            for (int ic = 0; ic < NC; ++ic) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dbc[ib * NC + ic];
-          atomicAdd(&Ko[(aoA + ia) * N + aoD + id], fk * s);
-          continue;
+          {  // K[b,d] += sum_{a,c} v D[a,c]
+            const int ib = r / ND, id = r - ib * ND;
+            double s = 0.0;
+            for (int ia = 0; ia < NA; ++ia)
+              for (int ic = 0; ic < NC; ++ic) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dac[ia * NC + ic];
+            atomicAdd(&Ko[(aoB + ib) * N + aoD + id], fk * s);
+          }
         }
-        r -= NA * ND;
-        if (!ab_neq) continue;
-        if (r < NB * NC) {  // K[b,c] += sum_{a,d} v D[a,d]
-          const int ib = r / NC, ic = r - ib * NC;
-          double s = 0.0;
-          for (int ia = 0; ia < NA; ++ia)
-            for (int id = 0; id < ND; ++id) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dad[ia * ND + id];
-          atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], fk * s);
-          continue;
-        }
-        r -= NB * NC;
-        if (!cd_neq) continue;
-        {  // K[b,d] += sum_{a,c} v D[a,c]
-          const int ib = r / ND, id = r - ib * ND;
-          double s = 0.0;
-          for (int ia = 0; ia < NA; ++ia)
-            for (int ic = 0; ic < NC; ++ic) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dac[ia * NC + ic];
-          atomicAdd(&Ko[(aoB + ib) * N + aoD + id], fk * s);
-        }
-      }
+      }  // densities
     }
   }
 }

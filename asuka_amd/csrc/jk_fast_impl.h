@@ -199,6 +199,7 @@ struct RowAcc {
   int kbc;  // [NB * colsC]
   int kbd;  // [NB * colsD] (same as kbc when lc == ld)
   int colsC, colsD;
+  int stride;  // offset of the second density's accumulators (two-density builds)
 };
 
 // One chunk of one shell quartet per lane; the bra pair P is the same on every
@@ -366,145 +367,150 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   // Sums over the chunk's own components are complete; the rest are partial
   // and add up across chunks. J[a,b] and the K rows of a and b go to the
   // workgroup's LDS accumulators when present (flushed once per row task),
-  // J[c,d] (distinct per lane) to global atomics.
+  // J[c,d] (distinct per lane) to global atomics. With a second density
+  // (two-density builds) the same tile is digested again into (J2, K2).
   const long long N = b.nao;
   const int aoA = b.shell_ao[A] + a0, aoB = b.shell_ao[B], aoC = b.shell_ao[C] + c0, aoD = b.shell_ao[Dsh];
   const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
   const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
   const double fk = bk ? 2.0 : 1.0;
-  const double* Dm = o.D;
-  double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
-  double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
   const double ks = (o.mode == MODE_FOCK) ? -0.5 * fk : fk;
   const bool lds = acc.on;  // uniform
   // compact columns of the lane's C (first component of the chunk) and D shells
   const int colC = lds ? b.shell_lao[C] + c0 : 0, colD = lds ? b.shell_lao[Dsh] : 0;
-  if (Jo) {
-    double dcd[NCDL];
+  const int nden = o.D2 ? 2 : 1;  // uniform
+  for (int di = 0; di < nden; ++di) {
+    const double* Dm = di ? o.D2 : o.D;
+    double* Jo = di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
+    double* Ko = di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+    const int so = di * acc.stride;  // this density's accumulators in s_acc
+    if (Jo) {
+      double dcd[NCDL];
 #pragma unroll
-    for (int ic = 0; ic < NCL; ++ic)
+      for (int ic = 0; ic < NCL; ++ic)
 #pragma unroll
-      for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = Dm[(aoC + ic) * N + aoD + id];
-    // The bra pair, hence the J[a,b] addresses, is the same on every lane:
-    // reduce across the wave, one update per element.
-    const unsigned long long amask = __ballot(active);
-    const int first = amask ? (__ffsll((unsigned long long)amask) - 1) : 0;
-    const int lane = threadIdx.x & 63;
+        for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = Dm[(aoC + ic) * N + aoD + id];
+      // The bra pair, hence the J[a,b] addresses, is the same on every lane:
+      // reduce across the wave, one update per element.
+      const unsigned long long amask = __ballot(active);
+      const int first = amask ? (__ffsll((unsigned long long)amask) - 1) : 0;
+      const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int r = 0; r < NAL * NB; ++r) {
-      double s = 0.0;
-#pragma unroll
-      for (int c = 0; c < NCDL; ++c) s = fma((double)tile[r * NCDL + c], dcd[c], s);
-      s = wave_sum(active ? fj * s : 0.0);
-      if (lane == first && active) {
-        if (lds)
-          atomicAdd(&s_acc[acc.jab + (a0 + r / NB) * NB + r % NB], s);
-        else
-          atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
-      }
-    }
-    if (bk && active) {
-      double dab[NAL * NB];
-#pragma unroll
-      for (int ia = 0; ia < NAL; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = Dm[(aoA + ia) * N + aoB + ib];
-#pragma unroll
-      for (int c = 0; c < NCDL; ++c) {
+      for (int r = 0; r < NAL * NB; ++r) {
         double s = 0.0;
 #pragma unroll
-        for (int r = 0; r < NAL * NB; ++r) s = fma((double)tile[r * NCDL + c], dab[r], s);
-        atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
+        for (int c = 0; c < NCDL; ++c) s = fma((double)tile[r * NCDL + c], dcd[c], s);
+        s = wave_sum(active ? fj * s : 0.0);
+        if (lane == first && active) {
+          if (lds)
+            atomicAdd(&s_acc[so + acc.jab + (a0 + r / NB) * NB + r % NB], s);
+          else
+            atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
+        }
       }
-    }
-  }
-  if (Ko && active) {
-    {  // K[a,c] += sum_{b,d} v D[b,d]
-      double dbd[NB * ND];
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = Dm[(aoB + ib) * N + aoD + id];
-#pragma unroll
-      for (int ia = 0; ia < NAL; ++ia)
-#pragma unroll
-        for (int ic = 0; ic < NCL; ++ic) {
-          double s = 0.0;
-#pragma unroll
-          for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-            for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbd[ib * ND + id], s);
-          if (lds)
-            atomicAdd(&s_acc[acc.kac + (a0 + ia) * acc.colsC + colC + ic], ks * s);
-          else
-            atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], ks * s);
-        }
-    }
-    if (cd_neq) {  // K[a,d] += sum_{b,c} v D[b,c]
-      double dbc[NB * NCL];
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int ic = 0; ic < NCL; ++ic) dbc[ib * NCL + ic] = Dm[(aoB + ib) * N + aoC + ic];
-#pragma unroll
-      for (int ia = 0; ia < NAL; ++ia)
-#pragma unroll
-        for (int id = 0; id < ND; ++id) {
-          double s = 0.0;
-#pragma unroll
-          for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-            for (int ic = 0; ic < NCL; ++ic) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbc[ib * NCL + ic], s);
-          if (lds)
-            atomicAdd(&s_acc[acc.kad + (a0 + ia) * acc.colsD + colD + id], ks * s);
-          else
-            atomicAdd(&Ko[(aoA + ia) * N + aoD + id], ks * s);
-        }
-    }
-    if (ab_neq) {  // K[b,c] += sum_{a,d} v D[a,d]
-      double dad[NAL * ND];
-#pragma unroll
-      for (int ia = 0; ia < NAL; ++ia)
-#pragma unroll
-        for (int id = 0; id < ND; ++id) dad[ia * ND + id] = Dm[(aoA + ia) * N + aoD + id];
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int ic = 0; ic < NCL; ++ic) {
-          double s = 0.0;
-#pragma unroll
-          for (int ia = 0; ia < NAL; ++ia)
-#pragma unroll
-            for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dad[ia * ND + id], s);
-          if (lds)
-            atomicAdd(&s_acc[acc.kbc + ib * acc.colsC + colC + ic], ks * s);
-          else
-            atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], ks * s);
-        }
-      if (cd_neq) {  // K[b,d] += sum_{a,c} v D[a,c]
-        double dac[NAL * NCL];
+      if (bk && active) {
+        double dab[NAL * NB];
 #pragma unroll
         for (int ia = 0; ia < NAL; ++ia)
 #pragma unroll
-          for (int ic = 0; ic < NCL; ++ic) dac[ia * NCL + ic] = Dm[(aoA + ia) * N + aoC + ic];
+          for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = Dm[(aoA + ia) * N + aoB + ib];
+#pragma unroll
+        for (int c = 0; c < NCDL; ++c) {
+          double s = 0.0;
+#pragma unroll
+          for (int r = 0; r < NAL * NB; ++r) s = fma((double)tile[r * NCDL + c], dab[r], s);
+          atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
+        }
+      }
+    }
+    if (Ko && active) {
+      {  // K[a,c] += sum_{b,d} v D[b,d]
+        double dbd[NB * ND];
 #pragma unroll
         for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = Dm[(aoB + ib) * N + aoD + id];
+#pragma unroll
+        for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+          for (int ic = 0; ic < NCL; ++ic) {
+            double s = 0.0;
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+              for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbd[ib * ND + id], s);
+            if (lds)
+              atomicAdd(&s_acc[so + acc.kac + (a0 + ia) * acc.colsC + colC + ic], ks * s);
+            else
+              atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], ks * s);
+          }
+      }
+      if (cd_neq) {  // K[a,d] += sum_{b,c} v D[b,c]
+        double dbc[NB * NCL];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int ic = 0; ic < NCL; ++ic) dbc[ib * NCL + ic] = Dm[(aoB + ib) * N + aoC + ic];
+#pragma unroll
+        for (int ia = 0; ia < NAL; ++ia)
 #pragma unroll
           for (int id = 0; id < ND; ++id) {
             double s = 0.0;
 #pragma unroll
-            for (int ia = 0; ia < NAL; ++ia)
+            for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-              for (int ic = 0; ic < NCL; ++ic)
-                s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dac[ia * NCL + ic], s);
+              for (int ic = 0; ic < NCL; ++ic) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbc[ib * NCL + ic], s);
             if (lds)
-              atomicAdd(&s_acc[acc.kbd + ib * acc.colsD + colD + id], ks * s);
+              atomicAdd(&s_acc[so + acc.kad + (a0 + ia) * acc.colsD + colD + id], ks * s);
             else
-              atomicAdd(&Ko[(aoB + ib) * N + aoD + id], ks * s);
+              atomicAdd(&Ko[(aoA + ia) * N + aoD + id], ks * s);
           }
       }
+      if (ab_neq) {  // K[b,c] += sum_{a,d} v D[a,d]
+        double dad[NAL * ND];
+#pragma unroll
+        for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) dad[ia * ND + id] = Dm[(aoA + ia) * N + aoD + id];
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int ic = 0; ic < NCL; ++ic) {
+            double s = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+              for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dad[ia * ND + id], s);
+            if (lds)
+              atomicAdd(&s_acc[so + acc.kbc + ib * acc.colsC + colC + ic], ks * s);
+            else
+              atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], ks * s);
+          }
+        if (cd_neq) {  // K[b,d] += sum_{a,c} v D[a,c]
+          double dac[NAL * NCL];
+#pragma unroll
+          for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+            for (int ic = 0; ic < NCL; ++ic) dac[ia * NCL + ic] = Dm[(aoA + ia) * N + aoC + ic];
+#pragma unroll
+          for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+              double s = 0.0;
+#pragma unroll
+              for (int ia = 0; ia < NAL; ++ia)
+#pragma unroll
+                for (int ic = 0; ic < NCL; ++ic)
+                  s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dac[ia * NCL + ic], s);
+              if (lds)
+                atomicAdd(&s_acc[so + acc.kbd + ib * acc.colsD + colD + id], ks * s);
+              else
+                atomicAdd(&Ko[(aoB + ib) * N + aoD + id], ks * s);
+            }
+        }
+      }
     }
-  }
+  }  // densities
 }
 
 template <class PL, int CH, typename CT>
@@ -575,6 +581,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
     }
   }
   const bool use_acc = w.acc_colsC > 0;
+  const int nden = o.D2 ? 2 : 1;  // densities digested (accumulator sets in s_acc)
   RowAcc acc{};
   int nacc = 0;
   if (use_acc) {  // row-task accumulators in s_acc, sized at launch
@@ -588,7 +595,8 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
     acc.kbc = acc.kac + NA * cC + (cD > 0 ? NA * cD : 0);
     acc.kbd = cD > 0 ? acc.kbc + NB * cC : acc.kbc;
     nacc = NA * NB + (NA + NB) * (cC + cD);
-    for (int t = threadIdx.x; t < nacc; t += nthr) s_acc[t] = 0.0;
+    acc.stride = nacc;
+    for (int t = threadIdx.x; t < nacc * nden; t += nthr) s_acc[t] = 0.0;
   }
   __syncthreads();
   const double* tab64 = kLds ? t64 : g64;
@@ -651,17 +659,18 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       const int A = b.sp_A[P], B = b.sp_B[P];
       const long long N = b.nao;
       const long long aoA = b.shell_ao[A], aoB = b.shell_ao[B];
-      double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
-      double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
       const int cC = w.acc_colsC, cD = w.acc_colsD;
       const int* laoC = b.lao_ao + w.lao_offC;
       const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD : w.lao_offC);
       const int cDe = cD > 0 ? cD : cC;
-      for (int t2 = threadIdx.x; t2 < nacc; t2 += nthr) {
+      for (int t2 = threadIdx.x; t2 < nacc * nden; t2 += nthr) {
         const double v = s_acc[t2];
         if (v == 0.0) continue;
         s_acc[t2] = 0.0;
-        int r = t2;
+        const bool second = t2 >= nacc;
+        double* Jo = second ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
+        double* Ko = second ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+        int r = second ? t2 - nacc : t2;
         if (r < NA * NB) {
           atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], v);
           continue;
@@ -689,7 +698,8 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
   }
 }
 
-// Bytes of LDS accumulators for a class pair with ket columns (colsC, colsD).
+// Bytes of LDS accumulators for a class pair with ket columns (colsC, colsD),
+// per density.
 inline size_t row_acc_bytes(int la, int lb, int colsC, int colsD) {
   const int NA = nc_(la), NB = nc_(lb);
   return sizeof(double) * (size_t)(NA * NB + (NA + NB) * (colsC + colsD));
@@ -701,7 +711,8 @@ hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, h
   const long long gmax = w.wpg == 8 ? 1024 : 2048;
   if (g > gmax) g = gmax;
   if (g < 1) g = 1;
-  const size_t lds = w.acc_colsC > 0 ? row_acc_bytes(PL::la, PL::lb, w.acc_colsC, w.acc_colsD) : 0;
+  const size_t lds =
+      w.acc_colsC > 0 ? (o.D2 ? 2 : 1) * row_acc_bytes(PL::la, PL::lb, w.acc_colsC, w.acc_colsD) : 0;
   if (lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute((const void*)eri_fast_kernel<PL>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

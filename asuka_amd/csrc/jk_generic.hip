@@ -89,16 +89,23 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     double* Dac = Dad + nA * nD;
     const int nblk = nAB + nCD + nA * nC + nA * nD + nB * nC + nB * nD;
     const bool digest = o.mode <= MODE_FOCK;
+    // two-density builds keep a second set (accumulators, density blocks) of
+    // the same layout 2 * nblk further on, filled from D2 and flushed to J2, K2
+    const int nden = (digest && o.D2) ? 2 : 1;
     if (digest) {
       const long long N = b.nao;
       const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
-      for (int e = lane; e < nblk; e += kWave) Lj_ab[e] = 0.0;
-      for (int e = lane; e < nCD; e += kWave) Dcd[e] = o.D[(aoC + e / nD) * N + aoD + e % nD];
-      for (int e = lane; e < nAB; e += kWave) Dab[e] = o.D[(aoA + e / nB) * N + aoB + e % nB];
-      for (int e = lane; e < nB * nD; e += kWave) Dbd[e] = o.D[(aoB + e / nD) * N + aoD + e % nD];
-      for (int e = lane; e < nB * nC; e += kWave) Dbc[e] = o.D[(aoB + e / nC) * N + aoC + e % nC];
-      for (int e = lane; e < nA * nD; e += kWave) Dad[e] = o.D[(aoA + e / nD) * N + aoD + e % nD];
-      for (int e = lane; e < nA * nC; e += kWave) Dac[e] = o.D[(aoA + e / nC) * N + aoC + e % nC];
+      for (int di = 0; di < nden; ++di) {
+        const double* Dm = di ? o.D2 : o.D;
+        const int so = di * 2 * nblk;
+        for (int e = lane; e < nblk; e += kWave) Lj_ab[so + e] = 0.0;
+        for (int e = lane; e < nCD; e += kWave) Dcd[so + e] = Dm[(aoC + e / nD) * N + aoD + e % nD];
+        for (int e = lane; e < nAB; e += kWave) Dab[so + e] = Dm[(aoA + e / nB) * N + aoB + e % nB];
+        for (int e = lane; e < nB * nD; e += kWave) Dbd[so + e] = Dm[(aoB + e / nD) * N + aoD + e % nD];
+        for (int e = lane; e < nB * nC; e += kWave) Dbc[so + e] = Dm[(aoB + e / nC) * N + aoC + e % nC];
+        for (int e = lane; e < nA * nD; e += kWave) Dad[so + e] = Dm[(aoA + e / nD) * N + aoD + e % nD];
+        for (int e = lane; e < nA * nC; e += kWave) Dac[so + e] = Dm[(aoA + e / nC) * N + aoC + e % nC];
+      }
       __syncthreads();
     }
 
@@ -223,13 +230,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
           continue;
         }
         const int ia = iab / nB, ib = iab - ia * nB, ic = icd / nD, id = icd - ic * nD;
-        atomicAdd(&Lj_ab[iab], v * Dcd[icd]);
-        if (bk) atomicAdd(&Lj_cd[icd], v * Dab[iab]);
-        atomicAdd(&Lk_ac[ia * nC + ic], v * Dbd[ib * nD + id]);
-        if (cd_neq) atomicAdd(&Lk_ad[ia * nD + id], v * Dbc[ib * nC + ic]);
-        if (ab_neq) {
-          atomicAdd(&Lk_bc[ib * nC + ic], v * Dad[ia * nD + id]);
-          if (cd_neq) atomicAdd(&Lk_bd[ib * nD + id], v * Dac[ia * nC + ic]);
+        for (int di = 0; di < nden; ++di) {
+          const int so = di * 2 * nblk;
+          atomicAdd(&Lj_ab[so + iab], v * Dcd[so + icd]);
+          if (bk) atomicAdd(&Lj_cd[so + icd], v * Dab[so + iab]);
+          atomicAdd(&Lk_ac[so + ia * nC + ic], v * Dbd[so + ib * nD + id]);
+          if (cd_neq) atomicAdd(&Lk_ad[so + ia * nD + id], v * Dbc[so + ib * nC + ic]);
+          if (ab_neq) {
+            atomicAdd(&Lk_bc[so + ib * nC + ic], v * Dad[so + ia * nD + id]);
+            if (cd_neq) atomicAdd(&Lk_bd[so + ib * nD + id], v * Dac[so + ia * nC + ic]);
+          }
         }
       }
     }
@@ -241,24 +251,33 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
       const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
       const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
       const double fk = (bk ? 2.0 : 1.0) * (o.mode == MODE_FOCK ? -0.5 : 1.0);
-      double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
-      double* Ko = (o.mode == MODE_FOCK) ? o.F : o.K;
-      if (Jo) {
-        for (int e = lane; e < nAB; e += kWave) atomicAdd(&Jo[(aoA + e / nB) * N + aoB + e % nB], fj * Lj_ab[e]);
-        if (bk)
-          for (int e = lane; e < nCD; e += kWave) atomicAdd(&Jo[(aoC + e / nD) * N + aoD + e % nD], fj * Lj_cd[e]);
-      }
-      if (Ko) {
-        for (int e = lane; e < nA * nC; e += kWave) atomicAdd(&Ko[(aoA + e / nC) * N + aoC + e % nC], fk * Lk_ac[e]);
-        if (cd_neq)
-          for (int e = lane; e < nA * nD; e += kWave)
-            atomicAdd(&Ko[(aoA + e / nD) * N + aoD + e % nD], fk * Lk_ad[e]);
-        if (ab_neq) {
-          for (int e = lane; e < nB * nC; e += kWave)
-            atomicAdd(&Ko[(aoB + e / nC) * N + aoC + e % nC], fk * Lk_bc[e]);
+      for (int di = 0; di < nden; ++di) {
+        double* Jo = di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
+        double* Ko = di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+        const double* L = Lj_ab + di * 2 * nblk;  // this density's block sums
+        const double* Ljab = L;
+        const double* Ljcd = Ljab + nAB;
+        const double* Lkac = Ljcd + nCD;
+        const double* Lkad = Lkac + nA * nC;
+        const double* Lkbc = Lkad + nA * nD;
+        const double* Lkbd = Lkbc + nB * nC;
+        if (Jo) {
+          for (int e = lane; e < nAB; e += kWave) atomicAdd(&Jo[(aoA + e / nB) * N + aoB + e % nB], fj * Ljab[e]);
+          if (bk)
+            for (int e = lane; e < nCD; e += kWave) atomicAdd(&Jo[(aoC + e / nD) * N + aoD + e % nD], fj * Ljcd[e]);
+        }
+        if (Ko) {
+          for (int e = lane; e < nA * nC; e += kWave) atomicAdd(&Ko[(aoA + e / nC) * N + aoC + e % nC], fk * Lkac[e]);
           if (cd_neq)
-            for (int e = lane; e < nB * nD; e += kWave)
-              atomicAdd(&Ko[(aoB + e / nD) * N + aoD + e % nD], fk * Lk_bd[e]);
+            for (int e = lane; e < nA * nD; e += kWave)
+              atomicAdd(&Ko[(aoA + e / nD) * N + aoD + e % nD], fk * Lkad[e]);
+          if (ab_neq) {
+            for (int e = lane; e < nB * nC; e += kWave)
+              atomicAdd(&Ko[(aoB + e / nC) * N + aoC + e % nC], fk * Lkbc[e]);
+            if (cd_neq)
+              for (int e = lane; e < nB * nD; e += kWave)
+                atomicAdd(&Ko[(aoB + e / nD) * N + aoD + e % nD], fk * Lkbd[e]);
+          }
         }
       }
       __syncthreads();  // LDS is reused by the next quartet
@@ -277,19 +296,25 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 
 }  // namespace
 
-size_t eri_generic_lds_bytes(int la, int lb, int lc, int ld) {
+size_t eri_generic_lds_bytes(int la, int lb, int lc, int ld, int nden) {
   const int nr = (la + lb + lc + ld) / 2 + 1;
   const int g1 = (la + lb + 1) * (lc + ld + 1);
   const int n1 = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
   const int nA = ncart_c(la), nB = ncart_c(lb), nC = ncart_c(lc), nD = ncart_c(ld);
   const int nblk = nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD;
-  return sizeof(double) * (2 * ASUKA_NROOTS_MAX + nr * 3 * g1 + nr * 3 * n1 + 2 * nblk);
+  return sizeof(double) * (2 * ASUKA_NROOTS_MAX + nr * 3 * g1 + nr * 3 * n1 + 2 * nden * nblk);
 }
 
 hipError_t launch_eri_generic(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {
   if (w.nwork <= 0) return hipSuccess;
   // LDS sized for the class (explicit lists may mix classes: then la..ld hold the maxima)
-  const size_t lds = eri_generic_lds_bytes(w.la, w.lb, w.lc, w.ld);
+  const int nden = (o.mode <= MODE_FOCK && o.D2) ? 2 : 1;
+  const size_t lds = eri_generic_lds_bytes(w.la, w.lb, w.lc, w.ld, nden);
+  if (lds > 64 * 1024) {  // (ff|ff) with two densities
+    const hipError_t e =
+        hipFuncSetAttribute((const void*)eri_generic_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   long long grid = w.nwork < 16384 ? w.nwork : 16384;
   hipLaunchKernelGGL(eri_generic_kernel, dim3((unsigned)grid), dim3(kWave), lds, s, b, w, o);
   return hipGetLastError();

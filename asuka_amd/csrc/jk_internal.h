@@ -130,6 +130,11 @@ struct EriOut {
   unsigned long long* counters;
   int mixed;          // allow the FP32 path
   double tau_mixed;   // FP32 when Q_ab Q_cd max|D| < tau_mixed for the whole wave
+  // MODE_JK with two densities (UHF/ROHF): each quartet is evaluated once and
+  // digested with D into (J, K) and with D2 into (J2, K2); D2 == nullptr for one
+  const double* D2;
+  double* J2;
+  double* K2;
 };
 
 constexpr int kCountStride = 16;

@@ -8,6 +8,7 @@ Drop-in mirror of the reference operator API ``asuka.hf.direct_jk``
 * ``direct_JK(ctx, D, *, want_J, want_K, profile, stats, tile_dtype,
   mixed_precision, f32_accum)`` (:1530) -> ``(J, K)``
 * ``direct_fock_rhf(ctx, D, hcore, ...)`` (:2205) -> ``F``
+* ``direct_JK_multi(ctx, Da, Db, ...)`` (:2716) -> ``(Ja, Ka, Jb, Kb)``
 
 Arrays are torch tensors on the GPU (float64, row-major (nao, nao)); numpy
 inputs are copied to the device. J, K and F are returned symmetrised. Shape
@@ -234,6 +235,64 @@ def direct_JK(
     return (J if want_J else None), (K if want_K else None)
 
 
+def direct_JK_multi(
+    ctx: DirectJKContext,
+    Da,
+    Db,
+    *,
+    want_J: bool = True,
+    want_K: bool = True,
+    profile: dict | None = None,
+    stats: dict | None = None,
+    tile_dtype: str | None = None,
+    mixed_precision: bool | None = None,
+    f32_accum: bool | None = None,
+    out: tuple | None = None,
+):
+    """(Ja, Ka, Jb, Kb) for two densities, each ERI evaluated once
+    (asuka/hf/direct_jk.py:2716-3095).
+
+    The UHF/ROHF build: every screened quartet is digested with Da into
+    (Ja, Ka) and with Db into (Jb, Kb) in the same kernel, so the cost is one
+    ERI pass instead of two. Returns symmetrised (nao, nao) float64 GPU
+    tensors (None for outputs not wanted); ``out=(Ja, Ka, Jb, Kb)`` reuses
+    caller buffers.
+    """
+    del tile_dtype, f32_accum
+    if not want_J and not want_K:
+        return None, None, None, None
+    torch = _torch()
+    nao = ctx.nao
+    Dad = _as_device_matrix(torch, Da, nao, "Da", ctx.device)
+    Dbd = _as_device_matrix(torch, Db, nao, "Db", ctx.device)
+
+    def _new():
+        return torch.empty((nao, nao), dtype=torch.float64, device=ctx.device)
+
+    if out is not None:
+        Ja, Ka, Jb, Kb = out
+    else:
+        Ja, Jb = (_new(), _new()) if want_J else (None, None)
+        Ka, Kb = (_new(), _new()) if want_K else (None, None)
+    if not want_J:
+        Ja = Jb = None
+    if not want_K:
+        Ka = Kb = None
+    flags = (_lib.WANT_J if want_J else 0) | (_lib.WANT_K if want_K else 0) | _flags(mixed_precision)
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().asuka_jk_build_multi(ctypes.c_void_p(ctx.handle), _lib.ptr(Dad), _lib.ptr(Dbd),
+                                               _lib.ptr(Ja), _lib.ptr(Ka), _lib.ptr(Jb), _lib.ptr(Kb), flags,
+                                               ctypes.c_void_p(_lib.stream_handle(ctx.device))))
+    if profile is not None:
+        torch.cuda.synchronize(ctx.device)
+        profile["direct_jk_t_s"] = float(time.perf_counter() - t0)
+        profile["direct_jk_ntasks"] = int(ctx.ntasks)
+    if stats is not None:
+        stats["direct_jk_ntasks"] = int(ctx.ntasks)
+        stats["classes"] = ctx.class_stats()
+    return Ja, Ka, Jb, Kb
+
+
 def direct_fock_rhf(
     ctx: DirectJKContext,
     D,
@@ -316,5 +375,5 @@ def eri_tiles(ctx: DirectJKContext, spab, spcd):
     return [host[o:o + s].reshape(shp) for o, s, shp in zip(off, sizes, shapes)]
 
 
-__all__ = ["DirectJKContext", "direct_JK", "direct_JK_batched", "direct_fock_rhf", "eri_tiles",
+__all__ = ["DirectJKContext", "direct_JK", "direct_JK_batched", "direct_JK_multi", "direct_fock_rhf", "eri_tiles",
            "make_direct_jk_context"]

@@ -15,6 +15,7 @@
  *   asuka/hf/direct_jk.py:1143  make_direct_jk_context  -> asuka_jk_ctx_create
  *   asuka/hf/direct_jk.py:1530  direct_JK               -> asuka_jk_build
  *   asuka/hf/direct_jk.py:2205  direct_fock_rhf         -> asuka_fock_build
+ *   asuka/hf/direct_jk.py:2716  direct_JK_multi         -> asuka_jk_build_multi
  *   asuka/cueri/screening.py:224 schwarz_shellpairs_device -> asuka_jk_schwarz
  *   asuka/cueri/gpu.py:6512     eri_rys_generic_device (tiles) -> asuka_jk_eri_tiles
  *   asuka/integrals/int1e_cart.py:1423 build_int1e_cart -> asuka_int1e
@@ -101,6 +102,16 @@ int asuka_jk_ctx_info(const asuka_jk_ctx* ctx, int64_t out[8]);
  * host synchronisation, no device allocation. Replaces direct_JK
  * (asuka/hf/direct_jk.py:1530-2202). J or K may be NULL when not wanted. */
 int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream);
+
+/* (Ja, Ka, Jb, Kb) for two symmetric densities (UHF/ROHF alpha and beta)
+ * from one evaluation of each shell quartet, digested with Da into (Ja, Ka)
+ * and with Db into (Jb, Kb); screening uses max(|Da|, |Db|) per shell-pair
+ * block. Outputs as for asuka_jk_build; with ASUKA_JK_WANT_J both Ja and Jb
+ * (with ASUKA_JK_WANT_K both Ka and Kb) must be distinct non-NULL buffers.
+ * Replaces direct_JK_multi (asuka/hf/direct_jk.py:2716-3095) and its
+ * two-density contraction (cueri_cuda_kernels_dense_scatter.cu:551-714). */
+int asuka_jk_build_multi(asuka_jk_ctx* ctx, const double* Da, const double* Db, double* Ja, double* Ka, double* Jb,
+                         double* Kb, int flags, void* stream);
 
 /* F = h + J - ½K, symmetrised. F must not alias D. Replaces direct_fock_rhf
  * (asuka/hf/direct_jk.py:2205-2713). */
