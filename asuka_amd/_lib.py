@@ -56,7 +56,8 @@ class JKOpts(ctypes.Structure):
         ("eps_schwarz", ctypes.c_double),
         ("eps_density", ctypes.c_double),
         ("tau_mixed", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 8),
+        ("streams", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 

@@ -35,6 +35,9 @@ static constexpr int kTaskUnits = 32;
 // LDS accumulators of one row task larger than this (or than the device allows
 // next to the root tables) fall back to global atomics.
 static constexpr size_t kRowAccMaxBytes = 72 * 1024;
+// streams a build's class-pair launches are spread over (the caller's included)
+static constexpr int kDefaultStreams = 4;
+static constexpr int kMaxStreams = 8;
 static constexpr size_t kRootTablesMaxBytes = 27 * 1024;
 // Accumulators up to this size run with 4-wave workgroups (16-unit tasks),
 // larger ones with 8-wave workgroups (32-unit tasks).
@@ -69,6 +72,8 @@ struct ClassPairHost {
   long long nquartets;  // Schwarz-screened quartets
   long long nprim_quartets;
   long long nprim_roots;
+  double cost;          // estimated work (EFM-v1 FLOPs before density screening)
+  int stream;           // 0: the caller's stream, k > 0: the context's side stream k - 1
 };
 
 __global__ void k_fill(double* x, long long n, double v) {
@@ -191,11 +196,21 @@ struct asuka_jk_ctx {
   unsigned long long* d_counters = nullptr;
   unsigned int* d_queue = nullptr;  // one work-queue head per class pair
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  // Side streams (default priority) over which the class-pair launches of a
+  // build are spread, forked from and joined back into the caller's stream
+  // by events, so one launch's tail overlaps the next launches.
+  std::vector<hipStream_t> side;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
+  std::vector<int> launch_order;  // class pairs, largest estimated cost first
   long long last_launches = 0;
   std::vector<void*> owned;
   ~asuka_jk_ctx() {
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (hipEvent_t e : ev_join) (void)hipEventDestroy(e);
+    for (hipStream_t st : side) (void)hipStreamDestroy(st);
     for (void* p : owned) (void)hipFree(p);
   }
 };
@@ -252,6 +267,18 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   ctx->eps_schwarz = opts->eps_schwarz;
   ctx->eps_density = opts->eps_density < 0.0 ? std::max(opts->eps_schwarz, 1e-10) : opts->eps_density;
   ctx->tau_mixed = opts->tau_mixed > 0.0 ? opts->tau_mixed : kDefaultTauMixed;
+  {
+    const int nst = opts->streams > 0 ? std::min(opts->streams, kMaxStreams) : kDefaultStreams;
+    HIP_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    for (int k = 1; k < nst; ++k) {
+      hipStream_t st = nullptr;
+      HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      ctx->side.push_back(st);
+      hipEvent_t e = nullptr;
+      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ctx->ev_join.push_back(e);
+    }
+  }
 
   // ---- shell pairs, l-ordered orientation (asuka/cueri/shell_pairs.py:72-116) ----
   for (int i = 0; i < nshell; ++i)
@@ -538,9 +565,29 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       if (cp.nseg == 0) continue;
       cp.nwork = padded;
       cp.nprim_roots = cp.nprim_quartets * nroots;
+      {  // EFM-v1 per primitive quartet and per quartet (digestion)
+        const int nmax = cp.la + cp.lb, mmax = cp.lc + cp.ld;
+        const double n1 = (double)(cp.la + 1) * (cp.lb + 1) * (cp.lc + 1) * (cp.ld + 1);
+        const double ncomp = (double)ncart_c(cp.la) * ncart_c(cp.lb) * ncart_c(cp.lc) * ncart_c(cp.ld);
+        const double per_prim = 24.0 + nroots * (24.0 + 15.0 * (nmax + 1) * (mmax + 1) + 6.0 * n1 + 4.0 * ncomp);
+        cp.cost = per_prim * (double)cp.nprim_quartets + 8.0 * ncomp * (double)cp.nquartets;
+      }
       ctx->cps.push_back(cp);
       ctx->ntasks += cp.nquartets;
     }
+  }
+  {  // class pairs over streams: largest first, each to the least loaded stream
+    const int nst = 1 + (int)ctx->side.size();
+    std::vector<int> order(ctx->cps.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ctx->cps[x].cost > ctx->cps[y].cost; });
+    std::vector<double> load(nst, 0.0);
+    for (int i : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      ctx->cps[i].stream = k;
+      load[k] += ctx->cps[i].cost;
+    }
+    ctx->launch_order = order;
   }
   if ((rc = dev_upload(&ctx->d_segs, segs, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_units, units, own, s))) return rc;
@@ -647,19 +694,30 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ctx->d_queue, nq);
     HIP_CHECK(hipGetLastError());
   }
+  const int nside = (int)ctx->side.size();
+  if (nside > 0) {  // fork: the side streams wait for everything before this build
+    HIP_CHECK(hipEventRecord(ctx->ev_fork, s));
+    for (hipStream_t st : ctx->side) HIP_CHECK(hipStreamWaitEvent(st, ctx->ev_fork, 0));
+  }
   long long n = 0;
-  for (size_t i = 0; i < ctx->cps.size(); ++i) {
-    ClassWork w = work_of(ctx, ctx->cps[i], (int)i);
+  for (int i : ctx->launch_order) {
+    const ClassPairHost& cph = ctx->cps[i];
+    hipStream_t ls = (cph.stream > 0 && cph.stream <= nside) ? ctx->side[cph.stream - 1] : s;
+    ClassWork w = work_of(ctx, cph, i);
     if (o.D2 && w.acc_colsC > 0) {  // two accumulator sets must fit the workgroup's LDS
       const size_t nA = ncart_c(w.la), nB = ncart_c(w.lb);
       const size_t bytes = 2 * sizeof(double) * (nA * nB + (nA + nB) * (size_t)(w.acc_colsC + w.acc_colsD));
       if (bytes > ctx->lds_per_group - kRootTablesMaxBytes) w.acc_colsC = w.acc_colsD = 0;
     }
     bool handled = false;
-    HIP_CHECK(launch_eri_fast(ctx->dev, w, o, s, &handled));
-    if (!handled) HIP_CHECK(launch_eri_coop(ctx->dev, w, o, s, &handled));
-    if (!handled) HIP_CHECK(launch_eri_generic(ctx->dev, w, o, s));
+    HIP_CHECK(launch_eri_fast(ctx->dev, w, o, ls, &handled));
+    if (!handled) HIP_CHECK(launch_eri_coop(ctx->dev, w, o, ls, &handled));
+    if (!handled) HIP_CHECK(launch_eri_generic(ctx->dev, w, o, ls));
     ++n;
+  }
+  for (int k = 0; k < nside; ++k) {  // join
+    HIP_CHECK(hipEventRecord(ctx->ev_join[k], ctx->side[k]));
+    HIP_CHECK(hipStreamWaitEvent(s, ctx->ev_join[k], 0));
   }
   if (timed) HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
   ctx->last_launches = n;

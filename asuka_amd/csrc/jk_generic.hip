@@ -117,6 +117,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
       atomicAdd(cnt + 8, (unsigned long long)nab * (unsigned long long)ncd);
     }
     const int npass = (ncomp + kWave * kNW - 1) / (kWave * kNW);
+    const bool single = nab * ncd == 1;
     const int oA = cart_off(la), oB = cart_off(lb), oC = cart_off(lc), oD = cart_off(ld);
 
     for (int pass = 0; pass < npass; ++pass) {
@@ -135,71 +136,76 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
       }
 
       for (int ipq = 0; ipq < nab * ncd; ++ipq) {
-        const int iab_p = ipq / ncd, icd_p = ipq - iab_p * ncd;
-        const PrimPair X = b.pp[ab0 + iab_p];
-        const PrimPair Y = b.pp[cd0 + icd_p];
-        const double p = X.eta, q = Y.eta;
-        const double inv_pq = 1.0 / (p + q);
-        const double dPQx = X.Px - Y.Px, dPQy = X.Py - Y.Py, dPQz = X.Pz - Y.Pz;
-        const double T = p * q * inv_pq * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz);
-        if (lane < nr) {
-          double xr, wr;
-          rys_root_dev(nr, lane, T, xr, wr);
-          // 2 pi^(5/2) / (p q sqrt(p+q)) cK_ab cK_cd
-          const double base = 34.98683665524972497 * X.inv_eta * Y.inv_eta * sqrt(inv_pq) * X.cK * Y.cK;
-          rw[lane] = xr;
-          rw[ASUKA_NROOTS_MAX + lane] = wr * base;
-        }
-        __syncthreads();
-        if (lane < 3 * nr) {
-          const int u = lane / 3, ax = lane - 3 * u;
-          const double xr = rw[u];
-          const double B0 = xr * 0.5 * inv_pq;
-          const double B1 = (1.0 - xr) * 0.5 * X.inv_eta + B0;
-          const double B1p = (1.0 - xr) * 0.5 * Y.inv_eta + B0;
-          const double Pa = sel3(ax, X.Px, X.Py, X.Pz), Qa = sel3(ax, Y.Px, Y.Py, Y.Pz);
-          const double Cc = (Pa - sel3(ax, Ax, Ay, Az)) + (q * inv_pq) * xr * (Qa - Pa);
-          const double Cp = (Qa - sel3(ax, Cx, Cy, Cz)) + (p * inv_pq) * xr * (Pa - Qa);
-          double* g = G + (u * 3 + ax) * g1;
-          const int s = mmax + 1;
-          g[0] = (ax == 2) ? rw[ASUKA_NROOTS_MAX + u] : 1.0;
-          if (nmax > 0) g[s] = Cc * g[0];
-          if (mmax > 0) g[1] = Cp * g[0];
-          for (int a = 2; a <= nmax; ++a) g[a * s] = B1 * (double)(a - 1) * g[(a - 2) * s] + Cc * g[(a - 1) * s];
-          for (int c = 2; c <= mmax; ++c) g[c] = B1p * (double)(c - 1) * g[c - 2] + Cp * g[c - 1];
-          if (nmax > 0 && mmax > 0) {
-            for (int a = 1; a <= nmax; ++a) {
-              g[a * s + 1] = (double)a * B0 * g[(a - 1) * s] + Cp * g[a * s];
-              for (int c = 2; c <= mmax; ++c)
-                g[a * s + c] = B1p * (double)(c - 1) * g[a * s + c - 2] + (double)a * B0 * g[(a - 1) * s + c - 1] +
-                               Cp * g[a * s + c - 1];
+        // with a single primitive quartet (uncontracted shells: every d/f
+        // class of these bases) the roots, G and I tables of the first pass
+        // stay valid in LDS for the remaining passes
+        if (!(single && pass > 0)) {
+          const int iab_p = ipq / ncd, icd_p = ipq - iab_p * ncd;
+          const PrimPair X = b.pp[ab0 + iab_p];
+          const PrimPair Y = b.pp[cd0 + icd_p];
+          const double p = X.eta, q = Y.eta;
+          const double inv_pq = 1.0 / (p + q);
+          const double dPQx = X.Px - Y.Px, dPQy = X.Py - Y.Py, dPQz = X.Pz - Y.Pz;
+          const double T = p * q * inv_pq * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz);
+          if (lane < nr) {
+            double xr, wr;
+            rys_root_dev(nr, lane, T, xr, wr);
+            // 2 pi^(5/2) / (p q sqrt(p+q)) cK_ab cK_cd
+            const double base = 34.98683665524972497 * X.inv_eta * Y.inv_eta * sqrt(inv_pq) * X.cK * Y.cK;
+            rw[lane] = xr;
+            rw[ASUKA_NROOTS_MAX + lane] = wr * base;
+          }
+          __syncthreads();
+          if (lane < 3 * nr) {
+            const int u = lane / 3, ax = lane - 3 * u;
+            const double xr = rw[u];
+            const double B0 = xr * 0.5 * inv_pq;
+            const double B1 = (1.0 - xr) * 0.5 * X.inv_eta + B0;
+            const double B1p = (1.0 - xr) * 0.5 * Y.inv_eta + B0;
+            const double Pa = sel3(ax, X.Px, X.Py, X.Pz), Qa = sel3(ax, Y.Px, Y.Py, Y.Pz);
+            const double Cc = (Pa - sel3(ax, Ax, Ay, Az)) + (q * inv_pq) * xr * (Qa - Pa);
+            const double Cp = (Qa - sel3(ax, Cx, Cy, Cz)) + (p * inv_pq) * xr * (Pa - Qa);
+            double* g = G + (u * 3 + ax) * g1;
+            const int s = mmax + 1;
+            g[0] = (ax == 2) ? rw[ASUKA_NROOTS_MAX + u] : 1.0;
+            if (nmax > 0) g[s] = Cc * g[0];
+            if (mmax > 0) g[1] = Cp * g[0];
+            for (int a = 2; a <= nmax; ++a) g[a * s] = B1 * (double)(a - 1) * g[(a - 2) * s] + Cc * g[(a - 1) * s];
+            for (int c = 2; c <= mmax; ++c) g[c] = B1p * (double)(c - 1) * g[c - 2] + Cp * g[c - 1];
+            if (nmax > 0 && mmax > 0) {
+              for (int a = 1; a <= nmax; ++a) {
+                g[a * s + 1] = (double)a * B0 * g[(a - 1) * s] + Cp * g[a * s];
+                for (int c = 2; c <= mmax; ++c)
+                  g[a * s + c] = B1p * (double)(c - 1) * g[a * s + c - 2] + (double)a * B0 * g[(a - 1) * s + c - 1] +
+                                 Cp * g[a * s + c - 1];
+              }
             }
           }
-        }
-        __syncthreads();
-        for (int e = lane; e < nr * 3 * n1; e += kWave) {
-          const int u = e / (3 * n1);
-          const int rem = e - u * 3 * n1;
-          const int ax = rem / n1;
-          int t = rem - ax * n1;
-          const int ia_ = t / sIa;
-          t -= ia_ * sIa;
-          const int ib_ = t / sIb;
-          t -= ib_ * sIb;
-          const int ic_ = t / sIc;
-          const int id_ = t - ic_ * sIc;
-          const double ab = sel3(ax, ABx, ABy, ABz), cd = sel3(ax, CDx, CDy, CDz);
-          const double* g = G + (u * 3 + ax) * g1;
-          const int s = mmax + 1;
-          double val = 0.0;
-          for (int n = 0; n <= ib_; ++n) {
-            double inner = 0.0;
-            for (int m = 0; m <= id_; ++m) inner += c_binom[id_][m] * ipow(cd, id_ - m) * g[(ia_ + n) * s + ic_ + m];
-            val += c_binom[ib_][n] * ipow(ab, ib_ - n) * inner;
+          __syncthreads();
+          for (int e = lane; e < nr * 3 * n1; e += kWave) {
+            const int u = e / (3 * n1);
+            const int rem = e - u * 3 * n1;
+            const int ax = rem / n1;
+            int t = rem - ax * n1;
+            const int ia_ = t / sIa;
+            t -= ia_ * sIa;
+            const int ib_ = t / sIb;
+            t -= ib_ * sIb;
+            const int ic_ = t / sIc;
+            const int id_ = t - ic_ * sIc;
+            const double ab = sel3(ax, ABx, ABy, ABz), cd = sel3(ax, CDx, CDy, CDz);
+            const double* g = G + (u * 3 + ax) * g1;
+            const int s = mmax + 1;
+            double val = 0.0;
+            for (int n = 0; n <= ib_; ++n) {
+              double inner = 0.0;
+              for (int m = 0; m <= id_; ++m) inner += c_binom[id_][m] * ipow(cd, id_ - m) * g[(ia_ + n) * s + ic_ + m];
+              val += c_binom[ib_][n] * ipow(ab, ib_ - n) * inner;
+            }
+            I[e] = val;
           }
-          I[e] = val;
+          __syncthreads();
         }
-        __syncthreads();
 #pragma unroll
         for (int j = 0; j < kNW; ++j) {
           double sacc = 0.0;

@@ -137,13 +137,16 @@ def make_direct_jk_context(
     tau_mixed: float = 0.0,
     atom_coords_bohr=None,
     atom_charges=None,
+    streams: int | None = None,
 ) -> DirectJKContext:
     """One-time setup (asuka/hf/direct_jk.py:1143-1470): shell pairs,
     primitive-pair tables, Schwarz bounds on the GPU, Q-sorted class work lists.
 
     ``eps_density`` defaults to ``$ASUKA_DIRECT_JK_EPS_DENSITY`` or
     max(eps_schwarz, 1e-10) (:1594-1596). Atom coordinates/charges are needed
-    only for the one-electron integrals.
+    only for the one-electron integrals. ``streams`` (default
+    ``$ASUKA_DIRECT_JK_STREAMS`` or 4) is the number of streams a build's
+    class-pair launches are spread over, the caller's included.
     """
     torch = _torch()
     del threads, max_tile_bytes, max_slab_tasks, gpu_task_budget_bytes
@@ -153,7 +156,10 @@ def make_direct_jk_context(
     if int(np.asarray(ao_basis.shell_l).size) == 0:
         raise ValueError("AO basis has zero shells")
     desc, keep = _basis_desc(ao_basis, atom_coords_bohr, atom_charges)
-    opts = _lib.JKOpts(float(eps_schwarz), float(eps_density), float(tau_mixed))
+    if streams is None:
+        env = os.environ.get("ASUKA_DIRECT_JK_STREAMS", "").strip()
+        streams = int(env) if env else 0
+    opts = _lib.JKOpts(float(eps_schwarz), float(eps_density), float(tau_mixed), int(streams))
     device = torch.device("cuda", torch.cuda.current_device())
     h = ctypes.c_void_p()
     _lib.check(_lib.lib().asuka_jk_ctx_create(ctypes.byref(desc), ctypes.byref(opts),

@@ -79,7 +79,11 @@ typedef struct {
                          bounds are all below tau_mixed runs the primitive setup, Rys roots,
                          recursion and transfer in FP32 (J/K accumulation stays FP64);
                          <= 0 => 1e-7 */
-  int32_t reserved[8];
+  int32_t streams;    /* a build's class-pair launches are spread over this many streams: the
+                         caller's and streams - 1 of the context's own, forked from and joined
+                         back into the caller's stream by events (stream order and graph capture
+                         are kept); <= 0 => 4, at most 8 */
+  int32_t reserved[7];
 } asuka_jk_opts;
 
 /* ABI version and last error message (thread-local). */
