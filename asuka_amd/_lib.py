@@ -72,6 +72,9 @@ SYMBOLS = [
     ("asuka_jk_build", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_jk_build_multi", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_fock_build", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    ("asuka_jk_autotune", ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    ("asuka_jk_tuning_get", ctypes.c_int, [_vp, _i32p, ctypes.c_int]),
+    ("asuka_jk_tuning_set", ctypes.c_int, [_vp, _i32p, ctypes.c_int, _vp]),
     ("asuka_jk_shell_pairs", ctypes.c_int, [_vp, _i32p, _i32p, _f64p]),
     ("asuka_jk_class_stats", ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
     ("asuka_jk_last_timing", ctypes.c_int, [_vp, _f64p, _i64p]),

@@ -25,6 +25,7 @@
 #include "rys_eval.h"
 
 hipError_t launch_eri_fast(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s, bool* handled);
+bool eri_fast_has_plan(int la, int lb, int lc, int ld);
 
 static thread_local std::string g_err;
 
@@ -72,9 +73,41 @@ struct ClassPairHost {
   long long nquartets;  // Schwarz-screened quartets
   long long nprim_quartets;
   long long nprim_roots;
-  double cost;          // estimated work (EFM-v1 FLOPs before density screening)
+  double cost;          // estimated work (EFM-v1 FLOPs before density screening), or measured ms once tuned
   int stream;           // 0: the caller's stream, k > 0: the context's side stream k - 1
+  int task_units;       // wave units per row task (at most)
+  std::vector<std::pair<int, int>> rows;  // (first unit relative to unit_start, units) per bra row
+  RowTask* d_tasks_own = nullptr;         // row tasks re-cut by the autotuner (owned by the context)
+  int task_units0 = 0;                    // the cut stored in the context's d_tasks
+  long long ntasks0 = 0;
+  std::vector<std::pair<int, std::pair<RowTask*, long long>>> cuts;  // other cuts made so far
+  double tuned_ms = 0.0;                  // time of the class pair's launch when tuned (0: not tuned)
 };
+
+// Row tasks of a class pair: each row's wave units cut into
+// ceil(nu / task_units) tasks of balanced size, longest (in primitive
+// quartets) first, since workgroups pull them from a queue in order and the
+// tail of the launch is then made of short tasks.
+static std::vector<RowTask> make_tasks(const ClassPairHost& cp, const WaveUnit* units, const std::vector<int>& col,
+                                       const std::vector<int>& pp_count, int task_units) {
+  std::vector<RowTask> tasks;
+  for (const auto& r : cp.rows) {
+    const int nu = r.second;
+    const int ntk = (nu + task_units - 1) / task_units;
+    for (int k = 0; k < ntk; ++k) {
+      const int b0 = (int)((long long)nu * k / ntk), b1 = (int)((long long)nu * (k + 1) / ntk);
+      tasks.push_back(RowTask{r.first + b0, b1 - b0});
+    }
+  }
+  std::stable_sort(tasks.begin(), tasks.end(), [&](const RowTask& x, const RowTask& y) {
+    const WaveUnit& ux = units[x.u0];
+    const WaveUnit& uy = units[y.u0];
+    const long long cx = (long long)x.nu * pp_count[ux.P] * pp_count[col[ux.col]];
+    const long long cy = (long long)y.nu * pp_count[uy.P] * pp_count[col[uy.col]];
+    return cx > cy;
+  });
+  return tasks;
+}
 
 __global__ void k_fill(double* x, long long n, double v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -181,6 +214,8 @@ struct asuka_jk_ctx {
   Segment* d_segs = nullptr;
   WaveUnit* d_units = nullptr;
   RowTask* d_tasks = nullptr;
+  std::vector<WaveUnit> h_units;  // host copies of the wave units and ket lists
+  std::vector<int> h_col;
   int lao_off[ASUKA_LMAX + 2] = {0};
   size_t lds_per_group = 64 * 1024;  // device limit of LDS per workgroup
   int nlao[ASUKA_LMAX + 1] = {0};
@@ -214,6 +249,35 @@ struct asuka_jk_ctx {
     for (void* p : owned) (void)hipFree(p);
   }
 };
+
+// Class pairs over streams, largest first, each to the least loaded stream
+// (longest-processing-time rule). The weights are the measured launch times
+// once every class pair is tuned, else the EFM-v1 estimates.
+static void assign_streams(asuka_jk_ctx* ctx) {
+  const int nst = 1 + (int)ctx->side.size();
+  bool tuned = !ctx->cps.empty();
+  for (const auto& cp : ctx->cps) tuned = tuned && cp.tuned_ms > 0.0;
+  auto weight = [&](int i) { return tuned ? ctx->cps[i].tuned_ms : ctx->cps[i].cost; };
+  std::vector<int> order(ctx->cps.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return weight(x) > weight(y); });
+  std::vector<double> load(nst, 0.0);
+  for (int i : order) {
+    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    ctx->cps[i].stream = k;
+    load[k] += weight(i);
+  }
+  ctx->launch_order = order;
+}
+
+// LDS row-accumulator columns of a class pair when they fit (else 0, 0).
+static bool row_acc_fits(const asuka_jk_ctx* ctx, int la, int lb, int lc, int ld, int* cC, int* cD) {
+  *cC = ctx->nlao[lc];
+  *cD = (lc == ld) ? 0 : ctx->nlao[ld];
+  const size_t bytes =
+      sizeof(double) * (size_t)(ncart_c(la) * ncart_c(lb) + (ncart_c(la) + ncart_c(lb)) * (*cC + *cD));
+  return bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - kRootTablesMaxBytes);
+}
 
 extern "C" {
 
@@ -493,6 +557,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
         cp.acc_colsD = fits ? cD : 0;
         cp.wpg = (fits && bytes > kRowAccSmallBytes) ? 8 : 4;
         task_units = cp.wpg == 8 ? kTaskUnits : kTaskUnits / 2;
+        cp.task_units = task_units;
       }
       long long padded = 0;
       for (const PairGroup& bg : groups[X]) {
@@ -519,13 +584,9 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
             offs.push_back(off);
             off += len;
             {
-              // the row's wave units, in row tasks of at most kTaskUnits
+              // the row's wave units (cut into row tasks below)
               const int nu = (int)((len + 63) / 64);
-              const int ntk = (nu + task_units - 1) / task_units;
-              for (int k = 0; k < ntk; ++k) {  // balanced task sizes
-                const int b0 = (int)((long long)nu * k / ntk), b1 = (int)((long long)nu * (k + 1) / ntk);
-                tasks.push_back(RowTask{(int)(units.size() - cp.unit_start) + b0, b1 - b0});
-              }
+              cp.rows.emplace_back((int)(units.size() - cp.unit_start), nu);
               for (long long c0 = 0; c0 < len; c0 += 64)
                 units.push_back(WaveUnit{P, kg.col_start + (int)c0, (int)std::min<long long>(64, len - c0), 0});
             }
@@ -551,16 +612,12 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       }
       cp.nseg = (int)segs.size() - cp.seg_start;
       cp.nunits = (long long)units.size() - cp.unit_start;
-      cp.ntasks = (long long)tasks.size() - cp.task_start;
-      // longest tasks first: the workgroups pull them from a queue, so the
-      // tail of the launch is made of short tasks
-      std::stable_sort(tasks.begin() + cp.task_start, tasks.end(), [&](const RowTask& x, const RowTask& y) {
-        const WaveUnit& ux = units[cp.unit_start + x.u0];
-        const WaveUnit& uy = units[cp.unit_start + y.u0];
-        const long long cx = (long long)x.nu * ctx->pp_count[ux.P] * ctx->pp_count[col_all[ux.col]];
-        const long long cy = (long long)y.nu * ctx->pp_count[uy.P] * ctx->pp_count[col_all[uy.col]];
-        return cx > cy;
-      });
+      {
+        const std::vector<RowTask> tk = make_tasks(cp, units.data() + cp.unit_start, col_all, ctx->pp_count, task_units);
+        tasks.insert(tasks.end(), tk.begin(), tk.end());
+        cp.ntasks = cp.ntasks0 = (long long)tk.size();
+        cp.task_units0 = task_units;
+      }
 
       if (cp.nseg == 0) continue;
       cp.nwork = padded;
@@ -576,25 +633,15 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       ctx->ntasks += cp.nquartets;
     }
   }
-  {  // class pairs over streams: largest first, each to the least loaded stream
-    const int nst = 1 + (int)ctx->side.size();
-    std::vector<int> order(ctx->cps.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ctx->cps[x].cost > ctx->cps[y].cost; });
-    std::vector<double> load(nst, 0.0);
-    for (int i : order) {
-      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      ctx->cps[i].stream = k;
-      load[k] += ctx->cps[i].cost;
-    }
-    ctx->launch_order = order;
-  }
+  assign_streams(ctx);
   if ((rc = dev_upload(&ctx->d_segs, segs, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_units, units, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_tasks, tasks, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_pair, row_pair_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_row_off, row_off_all, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_col_pair, col_all, own, s))) return rc;
+  ctx->h_units = std::move(units);  // kept for re-cutting row tasks (autotuner)
+  ctx->h_col = std::move(col_all);
   {
     std::vector<unsigned long long> cz(kCountStride * std::max<size_t>(1, ctx->cps.size()), 0ull);
     if ((rc = dev_upload(&ctx->d_counters, cz, own, s))) return rc;
@@ -672,7 +719,7 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   w.nwork = cp.nwork;
   w.units = ctx->d_units + cp.unit_start;
   w.nunits = cp.nunits;
-  w.tasks = ctx->d_tasks + cp.task_start;
+  w.tasks = cp.d_tasks_own ? cp.d_tasks_own : ctx->d_tasks + cp.task_start;
   w.ntasks = cp.ntasks;
   w.acc_colsC = cp.acc_colsC;
   w.acc_colsD = cp.acc_colsD;
@@ -681,6 +728,28 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   w.queue = ctx->d_queue + index;
   w.wpg = cp.wpg;
   return w;
+}
+
+// Fork: the context's side streams wait for everything issued on s so far.
+static int fork_streams(asuka_jk_ctx* ctx, hipStream_t s) {
+  if (ctx->side.empty()) return ASUKA_OK;
+  HIP_CHECK(hipEventRecord(ctx->ev_fork, s));
+  for (hipStream_t st : ctx->side) HIP_CHECK(hipStreamWaitEvent(st, ctx->ev_fork, 0));
+  return ASUKA_OK;
+}
+
+// Join: s waits for everything issued on the side streams.
+static int join_streams(asuka_jk_ctx* ctx, hipStream_t s) {
+  for (size_t k = 0; k < ctx->side.size(); ++k) {
+    HIP_CHECK(hipEventRecord(ctx->ev_join[k], ctx->side[k]));
+    HIP_CHECK(hipStreamWaitEvent(s, ctx->ev_join[k], 0));
+  }
+  return ASUKA_OK;
+}
+
+// The stream class pair cp is launched on.
+static hipStream_t stream_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, hipStream_t s) {
+  return (cp.stream > 0 && cp.stream <= (int)ctx->side.size()) ? ctx->side[cp.stream - 1] : s;
 }
 
 static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool timed) {
@@ -694,15 +763,12 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ctx->d_queue, nq);
     HIP_CHECK(hipGetLastError());
   }
-  const int nside = (int)ctx->side.size();
-  if (nside > 0) {  // fork: the side streams wait for everything before this build
-    HIP_CHECK(hipEventRecord(ctx->ev_fork, s));
-    for (hipStream_t st : ctx->side) HIP_CHECK(hipStreamWaitEvent(st, ctx->ev_fork, 0));
-  }
+  int rc = fork_streams(ctx, s);
+  if (rc) return rc;
   long long n = 0;
   for (int i : ctx->launch_order) {
     const ClassPairHost& cph = ctx->cps[i];
-    hipStream_t ls = (cph.stream > 0 && cph.stream <= nside) ? ctx->side[cph.stream - 1] : s;
+    hipStream_t ls = stream_of(ctx, cph, s);
     ClassWork w = work_of(ctx, cph, i);
     if (o.D2 && w.acc_colsC > 0) {  // two accumulator sets must fit the workgroup's LDS
       const size_t nA = ncart_c(w.la), nB = ncart_c(w.lb);
@@ -715,10 +781,7 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     if (!handled) HIP_CHECK(launch_eri_generic(ctx->dev, w, o, ls));
     ++n;
   }
-  for (int k = 0; k < nside; ++k) {  // join
-    HIP_CHECK(hipEventRecord(ctx->ev_join[k], ctx->side[k]));
-    HIP_CHECK(hipStreamWaitEvent(s, ctx->ev_join[k], 0));
-  }
+  if ((rc = join_streams(ctx, s))) return rc;
   if (timed) HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
   ctx->last_launches = n;
   return ASUKA_OK;
@@ -728,6 +791,7 @@ static inline unsigned grid_for(long long n, int block) {
   long long g = (n + block - 1) / block;
   return (unsigned)std::max<long long>(1, std::min<long long>(g, 65535));
 }
+
 
 // J/K of one density (D2 == nullptr) or of two densities from one evaluation
 // of each quartet; the outputs are zeroed first and symmetrised at the end.
@@ -838,6 +902,178 @@ int asuka_jk_build_batched(int nmol, asuka_jk_ctx* const* ctxs, const double* co
   return ASUKA_OK;
 }
 
+}  // extern "C"
+
+// Row tasks of class pair i re-cut to task_units units (device copy owned by
+// the context; the original cut stays in d_tasks).
+static int recut_tasks(asuka_jk_ctx* ctx, int i, int task_units, hipStream_t s) {
+  ClassPairHost& cp = ctx->cps[i];
+  if (task_units == cp.task_units) return ASUKA_OK;
+  cp.task_units = task_units;
+  if (task_units == cp.task_units0) {
+    cp.d_tasks_own = nullptr;
+    cp.ntasks = cp.ntasks0;
+    return ASUKA_OK;
+  }
+  for (const auto& c : cp.cuts)
+    if (c.first == task_units) {
+      cp.d_tasks_own = c.second.first;
+      cp.ntasks = c.second.second;
+      return ASUKA_OK;
+    }
+  const std::vector<RowTask> tk =
+      make_tasks(cp, ctx->h_units.data() + cp.unit_start, ctx->h_col, ctx->pp_count, task_units);
+  RowTask* d = nullptr;
+  HIP_CHECK(hipMalloc(&d, sizeof(RowTask) * std::max<size_t>(1, tk.size())));
+  ctx->owned.push_back(d);
+  HIP_CHECK(hipMemcpyAsync(d, tk.data(), sizeof(RowTask) * tk.size(), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  cp.d_tasks_own = d;
+  cp.ntasks = (long long)tk.size();
+  cp.cuts.push_back({task_units, {d, cp.ntasks}});
+  return ASUKA_OK;
+}
+
+// Applies (wavefronts per workgroup, units per task, LDS accumulators) to class pair i.
+static int apply_shape(asuka_jk_ctx* ctx, int i, int wpg, int task_units, bool acc, hipStream_t s) {
+  ClassPairHost& cp = ctx->cps[i];
+  int cC = 0, cD = 0;
+  const bool fits = row_acc_fits(ctx, cp.la, cp.lb, cp.lc, cp.ld, &cC, &cD);
+  cp.acc_colsC = (acc && fits) ? cC : 0;
+  cp.acc_colsD = (acc && fits) ? cD : 0;
+  cp.wpg = wpg;
+  return recut_tasks(ctx, i, task_units, s);
+}
+
+extern "C" {
+
+int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, void* stream) {
+  if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  if (reps < 1) return fail(ASUKA_E_INVALID, "reps must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  const long long nn = (long long)ctx->nao * ctx->nao;
+  double* scratch = nullptr;  // J and K of the timed launches (discarded)
+  HIP_CHECK(hipMalloc(&scratch, sizeof(double) * 2 * nn));
+  struct Free {
+    double* p;
+    ~Free() { (void)hipFree(p); }
+  } free_scratch{scratch};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  struct FreeEv {
+    hipEvent_t a, b;
+    ~FreeEv() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } free_ev{e0, e1};
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev.sp_A, ctx->dev.sp_B,
+                     ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
+  HIP_CHECK(hipGetLastError());
+  EriOut o{};
+  o.mode = MODE_JK;
+  o.D = D;
+  o.J = scratch;
+  o.K = scratch + nn;
+  o.eps_density = ctx->eps_density;
+  o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
+  o.tau_mixed = ctx->tau_mixed;
+  // median of reps timed launches (after one untimed) of class pair i as it is now set up
+  auto time_class = [&](int i, double* ms) -> int {
+    std::vector<float> t;
+    for (int r = 0; r <= reps; ++r) {
+      ClassWork w = work_of(ctx, ctx->cps[i], i);
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, w.queue, 1LL);
+      HIP_CHECK(hipEventRecord(e0, s));
+      bool handled = false;
+      HIP_CHECK(launch_eri_fast(ctx->dev, w, o, s, &handled));
+      if (!handled) HIP_CHECK(launch_eri_coop(ctx->dev, w, o, s, &handled));
+      if (!handled) HIP_CHECK(launch_eri_generic(ctx->dev, w, o, s));
+      HIP_CHECK(hipEventRecord(e1, s));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float f = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&f, e0, e1));
+      if (r > 0) t.push_back(f);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = (double)t[t.size() / 2];
+    return ASUKA_OK;
+  };
+  int rc = ASUKA_OK;
+  for (size_t ii = 0; ii < ctx->cps.size(); ++ii) {
+    const int i = (int)ii;
+    ClassPairHost& cp = ctx->cps[i];
+    double best = 0.0;
+    if ((rc = time_class(i, &best))) return rc;
+    if (eri_fast_has_plan(cp.la, cp.lb, cp.lc, cp.ld)) {
+      // candidate shapes: 4 or 8 wavefronts per workgroup, 8..64 wave units
+      // per row task, LDS row accumulators on (where they fit) or off
+      int cC = 0, cD = 0;
+      const bool fits = row_acc_fits(ctx, cp.la, cp.lb, cp.lc, cp.ld, &cC, &cD);
+      int bw = cp.wpg, bt = cp.task_units;
+      bool ba = cp.acc_colsC > 0;
+      for (int acc = fits ? 1 : 0; acc >= 0; --acc)
+        for (int wpg : {4, 8})
+          for (int tu : {8, 16, 32, 64}) {
+            if ((rc = apply_shape(ctx, i, wpg, tu, acc != 0, s))) return rc;
+            double ms = 0.0;
+            if ((rc = time_class(i, &ms))) return rc;
+            if (ms < best) {
+              best = ms;
+              bw = wpg;
+              bt = tu;
+              ba = acc != 0;
+            }
+          }
+      if ((rc = apply_shape(ctx, i, bw, bt, ba, s))) return rc;
+    }
+    cp.tuned_ms = std::max(best, 1e-6);
+  }
+  assign_streams(ctx);
+  return ASUKA_OK;
+}
+
+int asuka_jk_tuning_get(const asuka_jk_ctx* ctx, int32_t* rows, int cap) {
+  if (!ctx || (cap > 0 && !rows)) return fail(ASUKA_E_INVALID, "ctx and rows must be non-null");
+  const int n = std::min<int>(cap, (int)ctx->cps.size());
+  for (int i = 0; i < n; ++i) {
+    const ClassPairHost& cp = ctx->cps[i];
+    int32_t* r = rows + 8 * i;
+    r[0] = cp.la;
+    r[1] = cp.lb;
+    r[2] = cp.lc;
+    r[3] = cp.ld;
+    r[4] = cp.wpg;
+    r[5] = cp.task_units;
+    r[6] = cp.acc_colsC > 0 ? 1 : 0;
+    r[7] = (int32_t)std::llround(cp.tuned_ms * 1000.0);
+  }
+  return (int)ctx->cps.size();
+}
+
+int asuka_jk_tuning_set(asuka_jk_ctx* ctx, const int32_t* rows, int n, void* stream) {
+  if (!ctx || (n > 0 && !rows)) return fail(ASUKA_E_INVALID, "ctx and rows must be non-null");
+  hipStream_t s = (hipStream_t)stream;
+  for (int k = 0; k < n; ++k) {
+    const int32_t* r = rows + 8 * k;
+    if (r[4] != 4 && r[4] != 8) return fail(ASUKA_E_INVALID, "wavefronts per workgroup must be 4 or 8");
+    if (r[5] < 1 || r[5] > 1024) return fail(ASUKA_E_INVALID, "units per task must be in [1, 1024]");
+  }
+  for (int k = 0; k < n; ++k) {
+    const int32_t* r = rows + 8 * k;
+    for (size_t i = 0; i < ctx->cps.size(); ++i) {
+      ClassPairHost& cp = ctx->cps[i];
+      if (cp.la != r[0] || cp.lb != r[1] || cp.lc != r[2] || cp.ld != r[3]) continue;
+      const int rc = apply_shape(ctx, (int)i, r[4], r[5], r[6] != 0, s);
+      if (rc) return rc;
+      cp.tuned_ms = r[7] > 0 ? r[7] / 1000.0 : 0.0;
+    }
+  }
+  assign_streams(ctx);
+  return ASUKA_OK;
+}
+
 int asuka_jk_last_timing(asuka_jk_ctx* ctx, double* ms, int64_t* nlaunch) {
   if (!ctx || !ms) return fail(ASUKA_E_INVALID, "ctx and ms must be non-null");
   if (!ctx->ev_start || !ctx->ev_stop) return fail(ASUKA_E_INVALID, "no timed build recorded (use ASUKA_JK_TIME)");
@@ -900,13 +1136,16 @@ int asuka_jk_grad(asuka_jk_ctx* ctx, const double* D, const double* WJ, const do
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ctx->d_queue, nq);
     HIP_CHECK(hipGetLastError());
   }
-  for (size_t i = 0; i < ctx->cps.size(); ++i) {
-    ClassWork w = work_of(ctx, ctx->cps[i], (int)i);
+  int rc = fork_streams(ctx, s);  // class pairs spread over the context's streams, as in the build
+  if (rc) return rc;
+  for (int i : ctx->launch_order) {
+    hipStream_t ls = stream_of(ctx, ctx->cps[i], s);
+    ClassWork w = work_of(ctx, ctx->cps[i], i);
     bool handled = false;
-    HIP_CHECK(launch_eri_deriv_fast(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, ctx->natm, grad, s, &handled));
-    if (!handled) HIP_CHECK(launch_eri_deriv(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, grad, s));
+    HIP_CHECK(launch_eri_deriv_fast(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, ctx->natm, grad, ls, &handled));
+    if (!handled) HIP_CHECK(launch_eri_deriv(ctx->dev, w, D, WJ, WK, ctx->d_shell_atom, grad, ls));
   }
-  return ASUKA_OK;
+  return join_streams(ctx, s);
 }
 
 int asuka_int1e_grad(asuka_jk_ctx* ctx, const double* Dh, const double* Ws, double* grad, void* stream) {

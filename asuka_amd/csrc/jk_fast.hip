@@ -10,13 +10,20 @@ asuka_fast::Launcher find_launcher_p4(int la, int lb, int lc, int ld);
 asuka_fast::Launcher find_launcher_p5(int la, int lb, int lc, int ld);
 asuka_fast::Launcher find_launcher_p6(int la, int lb, int lc, int ld);
 
-hipError_t launch_eri_fast(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s, bool* handled) {
-  *handled = false;
-  if (w.q_ab || !w.units || (o.mode != MODE_JK && o.mode != MODE_FOCK)) return hipSuccess;
+static asuka_fast::Launcher find_fast(int la, int lb, int lc, int ld) {
   asuka_fast::Launcher f = nullptr;
   asuka_fast::Launcher (*finders[])(int, int, int, int) = {find_launcher_p0, find_launcher_p1, find_launcher_p2, find_launcher_p3, find_launcher_p4, find_launcher_p5, find_launcher_p6};
   for (auto fn : finders)
-    if (!f) f = fn(w.la, w.lb, w.lc, w.ld);
+    if (!f) f = fn(la, lb, lc, ld);
+  return f;
+}
+
+bool eri_fast_has_plan(int la, int lb, int lc, int ld) { return find_fast(la, lb, lc, ld) != nullptr; }
+
+hipError_t launch_eri_fast(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s, bool* handled) {
+  *handled = false;
+  if (w.q_ab || !w.units || (o.mode != MODE_JK && o.mode != MODE_FOCK)) return hipSuccess;
+  asuka_fast::Launcher f = find_fast(w.la, w.lb, w.lc, w.ld);
   if (!f) return hipSuccess;
   *handled = true;
   if (w.nunits <= 0) return hipSuccess;

@@ -91,6 +91,39 @@ class DirectJKContext:
                         "prim_quartets": int(r[5]), "prim_roots": int(r[6])})
         return out
 
+    def autotune(self, D, *, mixed_precision: bool | None = None, reps: int = 2) -> list[dict]:
+        """Time every class pair's launch on density D and keep the fastest
+        launch shape per class pair (see :meth:`tuning`); the class pairs are
+        then spread over the context's streams by their measured times.
+        Synchronous. Counterpart of the reference's online hot-class autotune
+        (asuka/cueri/gpu.py:4929-5005)."""
+        torch = _torch()
+        Dd = _as_device_matrix(torch, D, self.nao, "D", self.device)
+        _lib.check(_lib.lib().asuka_jk_autotune(ctypes.c_void_p(self.handle), _lib.ptr(Dd), _flags(mixed_precision),
+                                                int(reps), ctypes.c_void_p(_lib.stream_handle(self.device))))
+        return self.tuning()
+
+    def tuning(self) -> list[dict]:
+        """The tuning table: per class pair (la, lb, lc, ld), wavefronts per
+        workgroup, wave units per row task, LDS row accumulators on/off and the
+        tuned launch time in ms (0.0 if not tuned)."""
+        cap = 4096
+        rows = np.zeros((cap, 8), dtype=np.int32)
+        n = _lib.check(_lib.lib().asuka_jk_tuning_get(ctypes.c_void_p(self.handle),
+                                                      rows.ctypes.data_as(_lib._i32p), cap))
+        return [{"class": tuple(int(x) for x in r[:4]), "waves_per_group": int(r[4]), "units_per_task": int(r[5]),
+                 "lds_accumulators": bool(r[6]), "tuned_ms": int(r[7]) / 1000.0} for r in rows[:min(n, cap)]]
+
+    def set_tuning(self, table: list[dict]) -> None:
+        """Apply a tuning table (from :meth:`autotune` or :meth:`tuning`, possibly
+        of another molecule in the same basis); classes absent here are skipped."""
+        rows = np.array([list(t["class"]) + [int(t["waves_per_group"]), int(t["units_per_task"]),
+                                              int(bool(t["lds_accumulators"])), int(round(1000 * t["tuned_ms"]))]
+                         for t in table], dtype=np.int32).reshape(-1, 8)
+        rows = np.ascontiguousarray(rows)
+        _lib.check(_lib.lib().asuka_jk_tuning_set(ctypes.c_void_p(self.handle), rows.ctypes.data_as(_lib._i32p),
+                                                  int(rows.shape[0]), ctypes.c_void_p(_lib.stream_handle(self.device))))
+
 
 def _basis_desc(ao_basis, atom_xyz=None, atom_charge=None):
     keep = []

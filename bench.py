@@ -35,7 +35,8 @@ CONFIGS = {
     2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12,
             workload="(H2O)8 cc-pVTZ cart, one direct J/K build"),
     5: dict(n=64, seed=5, basis="cc-pVDZ", nocc=320, cseed=1005, eps=1e-12,
-            workload="(H2O)64 cc-pVDZ cart, one direct Fock build"),
+            workload="(H2O)64 cc-pVDZ cart, one direct Fock build",
+            workload_scf="(H2O)64 cc-pVDZ cart, 15 direct RHF iterations"),
     3: dict(nmol=32, seed=3, basis="cc-pVDZ", eps=1e-12,
             workload="32 water clusters of 1-16 molecules, cc-pVDZ cart, one batched J/K build"),
     4: dict(n=16, seed=4, basis="cc-pVDZ", eps=1e-12,
@@ -47,7 +48,8 @@ def run_batched(args, cfg) -> int:
     """Batched workload: 32 independent clusters (sizes default_rng(3).integers(1,
     17, 32), placement seeds 1000 + i), random orthonormal C with 5 orbitals per
     water (seed 2000 + i), one batched J/K build per step, eager and replayed
-    from a captured HIP graph."""
+    from a captured HIP graph, with the default launch shapes and then with
+    the shapes the autotuner picks on the largest molecule (applied to all)."""
     import torch
 
     from asuka_amd.basis import build_ao_basis_cart
@@ -71,30 +73,49 @@ def run_batched(args, cfg) -> int:
     log(f"batch: {len(ctxs)} molecules, {sum(c.nao for c in ctxs)} AOs, {nq} Schwarz-screened quartets "
         f"(contexts {t_ctx:.1f}s)")
     out = direct_JK_batched(ctxs, Ds)
-    for _ in range(max(0, args.warmup)):
-        direct_JK_batched(ctxs, Ds, out=out)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        direct_JK_batched(ctxs, Ds, out=out)
-    torch.cuda.synchronize(dev)
-    s_eager = (time.perf_counter() - t0) / args.steps
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        direct_JK_batched(ctxs, Ds, out=out)
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize(dev)
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph):
-        direct_JK_batched(ctxs, Ds, out=out)
-    graph.replay()
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+
+    def measure():
+        for _ in range(max(0, args.warmup)):
+            direct_JK_batched(ctxs, Ds, out=out)
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            direct_JK_batched(ctxs, Ds, out=out)
+        torch.cuda.synchronize(dev)
+        s_eager = (time.perf_counter() - t0) / args.steps
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            direct_JK_batched(ctxs, Ds, out=out)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize(dev)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            direct_JK_batched(ctxs, Ds, out=out)
         graph.replay()
-    torch.cuda.synchronize(dev)
-    s_graph = (time.perf_counter() - t0) / args.steps
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph.replay()
+        torch.cuda.synchronize(dev)
+        return s_eager, (time.perf_counter() - t0) / args.steps
+
+    s_eager0, s_graph0 = measure()
+    ref = [x.clone() for pair in out for x in pair if x is not None]
+    log(f"batch default shapes: eager {s_eager0 * 1e3:.2f} ms, graph {s_graph0 * 1e3:.2f} ms")
+    # autotune on the largest molecule, apply its table to the whole batch
+    big = int(np.argmax([c.nao for c in ctxs]))
+    t0 = time.perf_counter()
+    table = ctxs[big].autotune(Ds[big])
+    for i, c in enumerate(ctxs):
+        if i != big:
+            c.set_tuning(table)
+    t_tune = time.perf_counter() - t0
+    s_eager, s_graph = measure()
+    got = [x for pair in out for x in pair if x is not None]
+    dmax = max(float((a - b).abs().max()) for a, b in zip(got, ref))
+    log(f"batch autotuned shapes: eager {s_eager * 1e3:.2f} ms, graph {s_graph * 1e3:.2f} ms "
+        f"(tuning {t_tune:.2f}s, max |dJ|,|dK| vs default shapes {dmax:.1e})")
     res = {
         "metric": "batched J/K build: Schwarz-screened shell quartets/s over the batch (graph replay)",
         "value": nq / s_graph,
@@ -113,6 +134,95 @@ def run_batched(args, cfg) -> int:
         "s_per_batch_eager": s_eager,
         "s_per_batch_graph": s_graph,
         "graph_speedup": s_eager / s_graph,
+        "default_shapes": {"s_per_batch_eager": s_eager0, "s_per_batch_graph": s_graph0},
+        "autotuned_vs_default": s_graph0 / s_graph,
+        "autotune_s": t_tune,
+        "autotune_table_from": f"molecule {big} ({ctxs[big].nao} AOs), applied to all",
+        "max_abs_dJK_tuned_vs_default": dmax,
+        "context_build_s": t_ctx,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def run_scf(args, cfg) -> int:
+    """Large direct SCF: (H2O)64 cc-pVDZ (1600 AOs), 15 RHF iterations (core
+    guess, one predictor Fock build, DIIS) with the all-FP64 build and with the
+    mixed FP32/FP64 build, on one context; the energy after 15 iterations of
+    each is compared."""
+    import torch
+
+    from asuka_amd.frontend import build_int1e
+    from asuka_amd.basis import build_ao_basis_cart
+    from asuka_amd.hf.direct_jk import make_direct_jk_context
+    from asuka_amd.hf.direct_scf import rhf_direct
+    from asuka_amd.molecule import water_cluster
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    mol = water_cluster(cfg["n"], cfg["seed"], cfg["basis"])
+    ao, _ = build_ao_basis_cart(mol)
+    t0 = time.perf_counter()
+    ctx = make_direct_jk_context(ao, eps_schwarz=cfg["eps"], atom_coords_bohr=mol.coords_bohr,
+                                 atom_charges=mol.charges)
+    t_ctx = time.perf_counter() - t0
+    ints = build_int1e(ctx)
+    niter = 15
+    out = {}
+    kw = dict(nelec=mol.nelectron, enuc=float(mol.energy_nuc()))
+    D15 = None
+    for mode in ("fp64", "mixed"):
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        r = rhf_direct(ints.S, ints.hcore, ctx, max_cycle=niter, conv_tol=0.0, conv_tol_dm=0.0,
+                       mixed_precision=(mode == "mixed"), **kw)
+        torch.cuda.synchronize(dev)
+        out[mode] = {"s": time.perf_counter() - t0, "e_tot": r.e_tot, "iters": r.niter}
+        if mode == "fp64":
+            D15 = r.dm
+        log(f"{mode}: {niter} SCF iterations in {out[mode]['s']:.2f} s, E = {r.e_tot:.10f}")
+    dE = out["mixed"]["e_tot"] - out["fp64"]["e_tot"]
+    # precision of one build at the FP64 run's iteration-15 density:
+    # E = 1/2 sum D (h + F), so dE = 1/2 sum D (F_mixed - F_fp64)
+    from asuka_amd.hf.direct_jk import direct_fock_rhf
+
+    h = torch.as_tensor(np.asarray(ints.hcore), device=dev) if not isinstance(ints.hcore, torch.Tensor) else ints.hcore
+    F64 = direct_fock_rhf(ctx, D15, h)
+    F32 = direct_fock_rhf(ctx, D15, h, mixed_precision=True)
+    dE_build = float(0.5 * (D15 * (F32 - F64)).sum())
+    # converged energies, both modes continued from that density
+    conv = {}
+    for mode in ("fp64", "mixed"):
+        r = rhf_direct(ints.S, ints.hcore, ctx, max_cycle=60, conv_tol=1e-10, conv_tol_dm=1e-7, dm0=D15,
+                       mixed_precision=(mode == "mixed"), **kw)
+        conv[mode] = {"e_tot": r.e_tot, "iters": r.niter, "converged": bool(r.converged)}
+        log(f"{mode}: converged={r.converged} after {r.niter} more iterations, E = {r.e_tot:.10f}")
+    dE_conv = conv["mixed"]["e_tot"] - conv["fp64"]["e_tot"]
+    res = {
+        "metric": "direct RHF: seconds for 15 SCF iterations, mixed vs FP64 GPU build",
+        "value": out["mixed"]["s"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": out["mixed"]["s"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic (seeded water cluster geometry)",
+        "config": {"workload": cfg["workload_scf"], "nao": ao.nao, "basis": cfg["basis"], "n_water": cfg["n"],
+                   "geometry_seed": cfg["seed"], "eps_schwarz": cfg["eps"], "iterations": niter},
+        "fp64_s": out["fp64"]["s"],
+        "mixed_s": out["mixed"]["s"],
+        "speedup_mixed_vs_fp64": out["fp64"]["s"] / out["mixed"]["s"],
+        "e_tot_fp64": out["fp64"]["e_tot"],
+        "e_tot_mixed": out["mixed"]["e_tot"],
+        "dE_mixed_vs_fp64_after_15_iterations_Eh": dE,
+        "dE_mixed_vs_fp64_one_build_at_fp64_density_Eh": dE_build,
+        "dE_mixed_vs_fp64_converged_Eh": dE_conv,
+        "converged": conv,
+        "fock_builds_per_scf": niter + 1,
         "context_build_s": t_ctx,
     }
     print(json.dumps(res), flush=True)
@@ -262,6 +372,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fd", action="store_true", help="gradient workload: central-difference check on 6 coordinates")
+    ap.add_argument("--autotune", action="store_true", help="J/K workloads: autotune launch shapes before timing")
+    ap.add_argument("--scf", action="store_true", help="config 5: time 15 SCF iterations, FP64 and mixed")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is design-only "
@@ -280,6 +392,8 @@ def main() -> int:
         return run_gradient(args, cfg)
     if args.config == 3:
         return run_batched(args, cfg)
+    if args.config == 5 and args.scf:
+        return run_scf(args, cfg)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     mol = water_cluster(cfg["n"], cfg["seed"], cfg["basis"])
@@ -294,6 +408,12 @@ def main() -> int:
     log(f"context: nao={nao} nsp={ctx.nsp} schwarz-screened quartets={ctx.ntasks} ({t_ctx:.2f}s)")
 
     D = torch.as_tensor(D_h, device=dev)
+    t_tune = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        ctx.autotune(D)
+        t_tune = time.perf_counter() - t0
+        log(f"autotune: {t_tune:.2f}s")
     L = _lib.lib()
     h = ctypes.c_void_p(ctx.handle)
     stream = torch.cuda.current_stream(dev)
@@ -411,12 +531,14 @@ def main() -> int:
         "data": "synthetic (seeded water cluster geometry, random orthonormal occupied orbitals)",
         "config": {"workload": cfg["workload"], "nao": nao, "basis": cfg["basis"], "n_water": cfg["n"],
                    "geometry_seed": cfg["seed"], "density_seed": cfg["cseed"], "eps_schwarz": cfg["eps"],
-                   "eps_density": ctx.eps_density, "mode": main_mode},
+                   "eps_density": ctx.eps_density, "mode": main_mode,
+                   "launch_shapes": "autotuned" if t_tune is not None else "default"},
         "s_per_build": s_per_build,
         "quartets_per_build": work.quartets,
         "schwarz_screened_quartets": ctx.ntasks,
         "prim_quartet_roots_per_s": work.prim_roots / s_per_build,
         "context_build_s": t_ctx,
+        "autotune_s": t_tune,
         "mixed_vs_fp64": accuracy,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": metrics.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": metrics.roofline_fraction(f64, f32, kern_s), "traffic": None,

@@ -121,6 +121,24 @@ int asuka_jk_build_multi(asuka_jk_ctx* ctx, const double* Da, const double* Db, 
  * (asuka/hf/direct_jk.py:2205-2713). */
 int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, double* F, int flags, void* stream);
 
+/* Autotuner: times every class pair's kernel launch on density D (median of
+ * reps launches after one warm-up) and, for the class-specialised register
+ * kernels, every launch shape of {4, 8} wavefronts per workgroup x {8, 16, 32,
+ * 64} wave units (64 ket quartets each) per row task x LDS row accumulators
+ * on/off, keeps the fastest shape per class pair and spreads the class pairs
+ * over the context's streams by their measured times. Synchronous; allocates
+ * and frees scratch J/K. Counterpart of the reference's online hot-class
+ * autotune (asuka/cueri/gpu.py:4929-5005, ASUKA_CUERI_HOTCLASS_AUTOTUNE). */
+int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, void* stream);
+
+/* Tuning table, one row of 8 int32 per class pair: la, lb, lc, ld, wavefronts
+ * per workgroup, wave units per row task, LDS row accumulators (0/1), tuned
+ * launch time in microseconds (0: not tuned). get returns the number of class
+ * pairs (rows written: min(cap, that)); set applies the rows whose class
+ * exists in ctx (a table tuned on one molecule serves others of the same basis). */
+int asuka_jk_tuning_get(const asuka_jk_ctx* ctx, int32_t* rows, int cap);
+int asuka_jk_tuning_set(asuka_jk_ctx* ctx, const int32_t* rows, int n, void* stream);
+
 /* Shell pairs (l-ordered, asuka/cueri/shell_pairs.py:72-116) and their Schwarz
  * bounds Q_ab = sqrt(max (ab|ab)) (asuka/cueri/screening.py:224-314), host copies. */
 int asuka_jk_shell_pairs(const asuka_jk_ctx* ctx, int32_t* sp_A, int32_t* sp_B, double* Q);
@@ -159,8 +177,9 @@ int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream
 
 /* J/K of nmol independent molecules (one context each) in one call on one
  * stream: ctxs[m], D[m], J[m], K[m] as for asuka_jk_build (J or K may be NULL
- * arrays, or hold NULL entries). The builds are issued back to back without
- * host synchronisation, so the whole batch can be captured in one HIP graph.
+ * arrays, or hold NULL entries). The builds are issued in turn, each spread
+ * over its context's streams, without host synchronisation, so the whole
+ * batch can be captured in one HIP graph.
  * The replacement of the per-molecule loop over direct_JK calls (the batched
  * multi-molecule workload); the reference has no batched entry point. */
 int asuka_jk_build_batched(int nmol, asuka_jk_ctx* const* ctxs, const double* const* D, double* const* J,
