@@ -45,6 +45,8 @@ static constexpr size_t kRootTablesMaxBytes = 27 * 1024;
 static constexpr size_t kRowAccSmallBytes = 40 * 1024;
 // sqrt(2 pi^(5/2)), split evenly over the bra and ket primitive-pair factors
 static constexpr double kSqrt2Pi52 = 5.91496717279561287782;
+// primitive pairs with Q_p * max Q_p below this are dropped (with eps_schwarz > 0)
+static constexpr double kPrimPairEps = 1e-20;
 
 static int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -202,11 +204,13 @@ struct asuka_jk_ctx {
   int nshell = 0, nao = 0, nsp = 0, natm = 0, max_l = 0;
   double eps_schwarz = 0, eps_density = 0, tau_mixed = 0;
   std::vector<int> shell_l, shell_ao, shell_atom, sp_A, sp_B, pp_start, pp_count;
+  std::vector<int> pp_count0;  // primitive pairs per shell pair before primitive-pair screening
   std::vector<double> shell_xyz, Q, atom_xyz, atom_Z;
   std::vector<PrimPair> pp;
   std::vector<PrimPairF> ppf;
   std::vector<ClassPairHost> cps;
   long long ntasks = 0;
+  long long nprim_dropped = 0;  // primitive pairs removed by primitive-pair screening
   DevBasis dev{};
   int* d_row_pair = nullptr;
   long long* d_row_off = nullptr;
@@ -460,6 +464,94 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   dv.nao = ctx->nao;
   dv.nsp = ctx->nsp;
 
+  // ---- primitive-pair screening (only with Schwarz screening on) ----
+  // The Schwarz bound of every primitive pair alone, sqrt(max_i (p_i|p_i))
+  // including its coefficient product: one diagonal quartet per primitive
+  // pair, the generic kernel in Schwarz mode over a table in which each
+  // primitive pair is its own shell pair. A primitive pair whose bound times
+  // the largest bound is below kPrimPairEps adds less than that to every
+  // integral it enters (Cauchy-Schwarz per primitive quartet) and is dropped;
+  // these are mostly tight primitives on distant centres, whose
+  // exp(-a b / (a + b) |AB|^2) is tiny but not zero.
+  if (ctx->eps_schwarz > 0.0 && !ctx->pp.empty()) {
+    const int npp = (int)ctx->pp.size();
+    std::vector<void*> tmp;
+    struct FreeAll {
+      std::vector<void*>& v;
+      ~FreeAll() {
+        for (void* p : v) (void)hipFree(p);
+      }
+    } free_tmp{tmp};
+    std::vector<int> tA(npp), tB(npp), tS(npp), tC(npp, 1);
+    for (int sp = 0; sp < ctx->nsp; ++sp)
+      for (int t = ctx->pp_start[sp]; t < ctx->pp_start[sp] + ctx->pp_count[sp]; ++t) {
+        tA[t] = ctx->sp_A[sp];
+        tB[t] = ctx->sp_B[sp];
+        tS[t] = t;
+      }
+    int *d_tA, *d_tB, *d_tS, *d_tC;
+    double* d_Qp;
+    if ((rc = dev_upload(&d_tA, tA, tmp, s))) return rc;
+    if ((rc = dev_upload(&d_tB, tB, tmp, s))) return rc;
+    if ((rc = dev_upload(&d_tS, tS, tmp, s))) return rc;
+    if ((rc = dev_upload(&d_tC, tC, tmp, s))) return rc;
+    if ((rc = dev_upload(&d_Qp, std::vector<double>(npp, 0.0), tmp, s))) return rc;
+    DevBasis tb = dv;
+    tb.sp_A = d_tA;
+    tb.sp_B = d_tB;
+    tb.pp_start = d_tS;
+    tb.pp_count = d_tC;
+    tb.nsp = npp;
+    ClassWork w{};
+    w.la = w.lb = w.lc = w.ld = ctx->max_l;
+    w.nwork = npp;
+    w.q_ab = d_tS;  // the identity list: quartet t is (t|t)
+    w.q_cd = d_tS;
+    EriOut o{};
+    o.mode = MODE_SCHWARZ;
+    o.Qout = d_Qp;
+    HIP_CHECK(launch_eri_generic(tb, w, o, s));
+    std::vector<double> Qp(npp);
+    HIP_CHECK(hipMemcpyAsync(Qp.data(), d_Qp, sizeof(double) * npp, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    const double qmax = *std::max_element(Qp.begin(), Qp.end());
+    std::vector<PrimPair> pp2;
+    std::vector<PrimPairF> ppf2;
+    std::vector<int> start2(ctx->nsp), count2(ctx->nsp);
+    for (int sp = 0; sp < ctx->nsp; ++sp) {
+      start2[sp] = (int)pp2.size();
+      for (int t = ctx->pp_start[sp]; t < ctx->pp_start[sp] + ctx->pp_count[sp]; ++t)
+        if (Qp[t] * qmax >= kPrimPairEps) {
+          pp2.push_back(ctx->pp[t]);
+          ppf2.push_back(ctx->ppf[t]);
+        }
+      count2[sp] = (int)pp2.size() - start2[sp];
+    }
+    ctx->nprim_dropped = npp - (long long)pp2.size();
+    ctx->pp_count0 = ctx->pp_count;
+    if (ctx->nprim_dropped > 0) {
+      ctx->pp.swap(pp2);
+      ctx->ppf.swap(ppf2);
+      ctx->pp_start.swap(start2);
+      ctx->pp_count.swap(count2);
+      if ((rc = dev_upload(&d_pps, ctx->pp_start, own, s))) return rc;
+      if ((rc = dev_upload(&d_ppc, ctx->pp_count, own, s))) return rc;
+      if ((rc = dev_upload(&d_pp, ctx->pp, own, s))) return rc;
+      if ((rc = dev_upload(&d_ppf, ctx->ppf, own, s))) return rc;
+      dv.pp_start = d_pps;
+      dv.pp_count = d_ppc;
+      dv.pp = d_pp;
+      dv.ppf = d_ppf;
+    }
+  }
+
+  if (ctx->pp_count0.empty()) ctx->pp_count0 = ctx->pp_count;
+  {
+    int* d_ppc0;
+    if ((rc = dev_upload(&d_ppc0, ctx->pp_count0, own, s))) return rc;
+    dv.pp_count0 = d_ppc0;
+  }
+
   // ---- Schwarz bounds (device) ----
   ctx->Q.assign(ctx->nsp, 1.0);
   if (ctx->eps_schwarz > 0.0) {
@@ -498,15 +590,22 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
       cls_lb[pair_class(la, lb)] = lb;
     }
   // Pairs of each class are grouped by contraction depth (primitive pairs per
-  // shell pair), keeping the Q-descending order inside a group and each pair's
-  // rank in the full class list. A (bra group, ket group) segment then has
-  // uniform primitive-loop lengths; per bra pair its kets are a prefix of the
-  // ket group: Q_ket >= eps / Q_bra and, within one class, rank_ket <= rank_bra
-  // (each unordered pair of shell pairs once, asuka/hf/direct_jk.py:1228-1246).
+  // shell pair, in buckets no wider than 25 %), keeping the Q-descending order
+  // inside a group and each pair's rank in the full class list. The kets of a
+  // wavefront then have primitive loops of nearly equal length (each lane runs
+  // its own); per bra pair its kets are a prefix of the ket group:
+  // Q_ket >= eps / Q_bra and, within one class, rank_ket <= rank_bra (each
+  // unordered pair of shell pairs once, asuka/hf/direct_jk.py:1228-1246).
+  auto kbucket = [](int k) {
+    int b = 1;
+    while (b < k) b = std::max(b + 1, (b * 5) / 4);
+    return b;
+  };
   struct PairGroup {
-    int kcount;
+    int kcount;  // bucket (largest primitive-pair count of the group's pairs)
     std::vector<int> pairs;  // Q-descending
     std::vector<int> ranks;  // position in the full class list
+    std::vector<long long> kpref;  // prefix sums of the pairs' primitive-pair counts
     int col_start = 0;
   };
   std::vector<std::vector<PairGroup>> groups(ncls);
@@ -517,17 +616,19 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   std::vector<RowTask> tasks;
   for (int c = 0; c < ncls; ++c) {
     std::vector<int> kvals;
-    for (int sp : cls_pairs[c]) kvals.push_back(ctx->pp_count[sp]);
+    for (int sp : cls_pairs[c]) kvals.push_back(kbucket(ctx->pp_count[sp]));
     std::sort(kvals.begin(), kvals.end());
     kvals.erase(std::unique(kvals.begin(), kvals.end()), kvals.end());
     for (int kv : kvals) {
       PairGroup g;
       g.kcount = kv;
       for (size_t r = 0; r < cls_pairs[c].size(); ++r)
-        if (ctx->pp_count[cls_pairs[c][r]] == kv) {
+        if (kbucket(ctx->pp_count[cls_pairs[c][r]]) == kv) {
           g.pairs.push_back(cls_pairs[c][r]);
           g.ranks.push_back((int)r);
         }
+      g.kpref.assign(1, 0);
+      for (int sp : g.pairs) g.kpref.push_back(g.kpref.back() + ctx->pp_count[sp]);
       g.col_start = (int)col_all.size();
       col_all.insert(col_all.end(), g.pairs.begin(), g.pairs.end());
       groups[c].push_back(std::move(g));
@@ -607,7 +708,10 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
           row_off_all.push_back(off);
           padded += (off + 63) / 64 * 64;
           cp.nquartets += off;
-          cp.nprim_quartets += (long long)bg.kcount * kg.kcount * off;
+          for (size_t r = 0; r < rowp.size(); ++r) {  // primitive quartets evaluated
+            const long long len = (r + 1 < rowp.size() ? offs[r + 1] : off) - offs[r];
+            cp.nprim_quartets += (long long)ctx->pp_count[rowp[r]] * (kg.kpref[len] - kg.kpref[0]);
+          }
         }
       }
       cp.nseg = (int)segs.size() - cp.seg_start;

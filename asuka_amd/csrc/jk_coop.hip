@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
       if (o.counters && lane == 0) {
         unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
         atomicAdd(cnt + (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp)), 1ull);
-        atomicAdd(cnt + 8, (unsigned long long)nab * (unsigned long long)ncd);
+        atomicAdd(cnt + 8, (unsigned long long)b.pp_count0[P] * (unsigned long long)b.pp_count0[Qp]);
       }
       __syncthreads();  // the previous quartet's digestion is done with the LDS
       for (int e = lane; e < K::NCOMP; e += kWave) tile[e] = 0.0;

@@ -631,7 +631,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
         const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
         unsigned long long* cnt = o.counters + kCountStride * w.cp_index;
         const unsigned long long kq =
-            active ? (unsigned long long)b.pp_count[P] * (unsigned long long)b.pp_count[Qp] : 0ull;
+            active ? (unsigned long long)b.pp_count0[P] * (unsigned long long)b.pp_count0[Qp] : 0ull;
         const int bin = (((A != B) << 2) | ((C != Dsh) << 1) | (P != Qp));
         for (int k = 0; k < 8; ++k) {
           const unsigned long long m = __ballot(active && bin == k);

@@ -49,6 +49,7 @@ struct DevBasis {
   const int* sp_B;
   const int* pp_start;       // (nsp) first primitive pair of the shell pair
   const int* pp_count;       // (nsp)
+  const int* pp_count0;      // (nsp) primitive pairs before primitive-pair screening (work statistics)
   const PrimPair* pp;
   const PrimPairF* ppf;      // same indexing as pp
   const int* shell_lao;      // (nshell) first AO of the shell in the compact numbering of its l
@@ -59,9 +60,10 @@ struct DevBasis {
   int nsp;
 };
 
-// Work of one class pair. Quartets are grouped into segments of equal bra and
-// ket contraction depth (primitive pairs per shell pair), so the quartets of a
-// segment share both primitive-loop lengths. In a segment, rows are bra pairs
+// Work of one class pair. Quartets are grouped into segments by bra and ket
+// contraction depth (primitive pairs per shell pair, in buckets at most 25 %
+// wide), so the quartets of a segment have nearly equal primitive-loop
+// lengths. In a segment, rows are bra pairs
 // (Q-descending) and each row runs over a prefix of the segment's ket list
 // (Q-descending). Segment starts are padded to multiples of 64 in the class
 // pair's work index space, so a wavefront never spans two segments; padding
@@ -72,7 +74,7 @@ struct Segment {
   int row_start;   // into row_pair / row_off (row_off has nrows + 1 entries)
   int nrows;
   int col_start;   // into col_pair
-  int kab, kcd;    // primitive pairs of every bra / ket pair in the segment
+  int kab, kcd;    // largest primitive-pair count of the segment's bra / ket pairs (bucket)
   int pad;
 };
 
