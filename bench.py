@@ -193,7 +193,7 @@ def run_scf(args, cfg) -> int:
     # converged energies, both modes continued from that density
     conv = {}
     for mode in ("fp64", "mixed"):
-        r = rhf_direct(ints.S, ints.hcore, ctx, max_cycle=60, conv_tol=1e-10, conv_tol_dm=1e-7, dm0=D15,
+        r = rhf_direct(ints.S, ints.hcore, ctx, max_cycle=60, conv_tol=1e-9, conv_tol_dm=1e-6, dm0=D15,
                        mixed_precision=(mode == "mixed"), **kw)
         conv[mode] = {"e_tot": r.e_tot, "iters": r.niter, "converged": bool(r.converged)}
         log(f"{mode}: converged={r.converged} after {r.niter} more iterations, E = {r.e_tot:.10f}")
