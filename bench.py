@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Fock-build s & quartets/s: mixed vs FP64 GPU at ≤1e-6 Eh; gradient s/eval"
 
+# HBM bytes of one build's ERI kernels (FETCH_SIZE + WRITE_SIZE, each scaled
+# by what the counter reads for a 1 GiB device copy), measured with rocprofv3
+# --pmc on the FP64 build: profiles/jk_build_h2o8_ccpvtz_hbm_traffic.md.
+HBM_TRAFFIC = {2: 2.204e10}
+
 CONFIGS = {
     2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12,
             workload="(H2O)8 cc-pVTZ cart, one direct J/K build"),
@@ -374,6 +379,8 @@ def main() -> int:
     ap.add_argument("--fd", action="store_true", help="gradient workload: central-difference check on 6 coordinates")
     ap.add_argument("--autotune", action="store_true", help="J/K workloads: autotune launch shapes before timing")
     ap.add_argument("--scf", action="store_true", help="config 5: time 15 SCF iterations, FP64 and mixed")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (the calibration kernel of the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is design-only "
@@ -381,6 +388,17 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_probe > 0:
+        # a plain copy of known size: reads and writes copy_probe bytes once
+        n = args.copy_probe // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8}), flush=True)
+        return 0
 
     from asuka_amd import _lib, metrics
     from asuka_amd.basis import build_ao_basis_cart
@@ -541,7 +559,7 @@ def main() -> int:
         "autotune_s": t_tune,
         "mixed_vs_fp64": accuracy,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": metrics.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": metrics.roofline_fraction(f64, f32, kern_s), "traffic": None,
+                     "frac": metrics.roofline_fraction(f64, f32, kern_s), "traffic": HBM_TRAFFIC.get(args.config),
                      "flops_fp64": f64, "flops_fp32": f32,
                      "flops_per_build_efm_v1": work.flops, "kernel_ms_per_build": kern_s * 1e3,
                      "kernel_launches_per_build": int(nl.value),
