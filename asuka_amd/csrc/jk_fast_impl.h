@@ -256,7 +256,8 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
       // 2 pi^(5/2) cK_ab cK_cd / (p q sqrt(p+q))
       const CT base = CT(X.ck) * ckq * rsq_nr(spq);
       CT rx[NR], rw[NR];
-      rys_roots_blk<NR, CT>(T, tab, rx, rw);
+      rys_roots_blk<NR, CT, (NR <= kLdsRootsMax ? rys_c::lds_stride(NR, sizeof(CT)) : rys_c::interval_size(NR))>(
+          T, tab, rx, rw);
       const CT qf = q * inv_pq, pf = p * inv_pq;
 #pragma unroll
       for (int u = 0; u < NR; ++u) {
@@ -568,16 +569,21 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
   constexpr int NCH = PL::NCH, NR = PL::NR;
   constexpr int NA = nc_(PL::la), NB = PL::NB;
   constexpr bool kLds = NR <= kLdsRootsMax;
+  // LDS copies of the root tables, intervals padded (rys_c::lds_stride)
+  constexpr int NFI = rys_c::interval_size(NR);
+  constexpr int S64 = rys_c::lds_stride(NR, 8), S32 = rys_c::lds_stride(NR, 4);
   constexpr int TB = kLds ? rys_c::block_size(NR) : 1;
+  constexpr int TB64 = kLds ? rys_c::nint[NR] * S64 : 1, TB32 = kLds ? rys_c::nint[NR] * S32 : 1;
   const int wpg = w.wpg, nthr = 64 * w.wpg;  // the launch's block size (uniform)
-  __shared__ double t64[TB];
-  __shared__ float t32[TB];
+  __shared__ double t64[TB64];
+  __shared__ float t32[TB32];
   const double* g64 = rys_dev::rys_coef + rys_c::coef_off[NR];
   const float* g32 = rys_dev_f::rys_coef + rys_c::coef_off[NR];
   if constexpr (kLds) {
     for (int t = threadIdx.x; t < TB; t += nthr) {
-      t64[t] = g64[t];
-      t32[t] = g32[t];
+      const int iv = t / NFI, f = t - iv * NFI;
+      t64[iv * S64 + f] = g64[t];
+      t32[iv * S32 + f] = g32[t];
     }
   }
   const bool use_acc = w.acc_colsC > 0;

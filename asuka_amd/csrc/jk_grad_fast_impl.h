@@ -208,7 +208,7 @@ __device__ __forceinline__ void deriv_unit(const DevBasis& b, const GradFastArgs
       const double base = 34.98683665524972497 * X.inv_eta * Y.inv_eta * rsq_nr(spq) * X.cK * Y.cK;
       const double PA[3] = {X.Px - cA[0], X.Py - cA[1], X.Pz - cA[2]};
       double rx[NR], rwt[NR];
-      rys_roots_blk<NR, double>(T, tab, rx, rwt);
+      rys_roots_blk<NR, double, (NR <= 3 ? rys_c::lds_stride(NR, 8) : rys_c::interval_size(NR))>(T, tab, rx, rwt);
 #pragma unroll
       for (int u = 0; u < NR; ++u) {
         const double x = rx[u];
@@ -290,12 +290,13 @@ __global__ __launch_bounds__(kThreads) void eri_deriv_fast_kernel(DevBasis b, Cl
   constexpr int NR = DPlan<LA, LB, LC, LD>::NR;
   constexpr bool kLds = NR <= 3;
   constexpr int TB = kLds ? rys_c::block_size(NR) : 1;
-  __shared__ double t64[TB];
+  constexpr int NFI = rys_c::interval_size(NR), S64 = rys_c::lds_stride(NR, 8);  // padded LDS intervals
+  __shared__ double t64[kLds ? rys_c::nint[NR] * S64 : 1];
   __shared__ double sgrad[3 * kLdsGradMaxAtoms];
   __shared__ int s_task;
   const double* g64 = rys_dev::rys_coef + rys_c::coef_off[NR];
   if constexpr (kLds)
-    for (int t = threadIdx.x; t < TB; t += kThreads) t64[t] = g64[t];
+    for (int t = threadIdx.x; t < TB; t += kThreads) t64[(t / NFI) * S64 + t % NFI] = g64[t];
   const bool lds_grad = g.natm <= kLdsGradMaxAtoms;
   if (lds_grad)
     for (int t = threadIdx.x; t < 3 * g.natm; t += kThreads) sgrad[t] = 0.0;

@@ -132,6 +132,18 @@ constexpr double tmax[] = {RYS_TMAX_LIST};
 constexpr int coef_off[] = {RYS_COEF_OFF_LIST};
 // coefficients of the n-root block
 constexpr int block_size(int n) { return nint[n] * 2 * n * (RYS_DEG + 1); }
+// coefficients of one interval of the n-root block
+constexpr int interval_size(int n) { return 2 * n * (RYS_DEG + 1); }
+// Interval stride (elements of size esize) of an LDS copy of the n-root block:
+// one interval's coefficients padded to an odd number of 16-byte slots, so
+// lanes whose T falls in different intervals read different LDS banks with
+// their 16-byte reads (same-interval lanes share addresses and broadcast).
+constexpr int lds_stride(int n, int esize) {
+  const int per16 = 16 / esize;
+  int s = (interval_size(n) + per16 - 1) / per16 * per16;
+  if ((s / per16) % 2 == 0) s += per16;
+  return s;
+}
 static_assert(RYS_H == 2.0, "the block evaluators assume intervals of width 2");
 }  // namespace rys_c
 
@@ -161,10 +173,11 @@ __device__ __forceinline__ float rsq_nr(float x) { return __builtin_amdgcn_rsqf(
 // All N roots and weights at T from the N-root coefficient block `blk`
 // (rys_coef + coef_off[N], or a copy of it in LDS), in the arithmetic of CT.
 // At T >= tmax[N] the Gauss-Hermite limit, with 1/T and 1/sqrt(T) in CT.
-template <int N, typename CT>
+// STRIDE is the interval stride of blk: interval_size(N) for the global
+// table, lds_stride(N, sizeof(CT)) for a padded LDS copy.
+template <int N, typename CT, int STRIDE = 2 * N * (RYS_DEG + 1)>
 __device__ __forceinline__ void rys_roots_blk(CT T, const CT* __restrict__ blk, CT* x, CT* w) {
   constexpr int NI = rys_c::nint[N];
-  constexpr int NF = 2 * N * (RYS_DEG + 1);
   if (T >= CT(rys_c::tmax[N])) {
     const CT it = rcp_nr(T);
     const CT sit = rsq_nr(T);
@@ -178,7 +191,7 @@ __device__ __forceinline__ void rys_roots_blk(CT T, const CT* __restrict__ blk, 
   int i = (int)(T * CT(0.5));
   i = i < NI - 1 ? i : NI - 1;
   const CT s = (T - CT(2 * i)) - CT(1);
-  const CT* c = blk + i * NF;
+  const CT* c = blk + i * STRIDE;
 #pragma unroll
   for (int f = 0; f < 2 * N; ++f) {
     const CT* cf = c + f * (RYS_DEG + 1);

@@ -8,8 +8,10 @@ asuka/frontend/_scf_dispatch.py:147-184 -> asuka/frontend/scf.py:451-498
 integrals, direct J/K context, ``rhf_direct``, result with
 ``two_e_backend="direct"`` and ``direct_jk_ctx``.
 
-Only the integral-direct RHF path exists here; other two-electron backends and
-open-shell methods raise ``NotImplementedError``. One-electron integrals are
+Open-shell methods follow the same branch to ``run_uhf_direct`` /
+``run_rohf_direct`` (asuka/frontend/_scf_dispatch.py:178-183, asuka/frontend/
+scf.py:502-) over the two-density build. Other two-electron backends (DF,
+THC, dense) and DFT raise ``NotImplementedError``. One-electron integrals are
 computed on the GPU by the same library.
 """
 
@@ -23,7 +25,7 @@ import numpy as np
 
 from .basis import build_ao_basis_cart
 from .hf.direct_jk import make_direct_jk_context
-from .hf.direct_scf import SCFResult, rhf_direct
+from .hf.direct_scf import SCFResult, rhf_direct, rohf_direct, uhf_direct
 
 
 @dataclass(frozen=True)
@@ -103,12 +105,59 @@ def run_rhf_direct(mol, *, basis: Any | None = None, eps_schwarz: float = 1e-12,
                           direct_jk_ctx=jk_ctx)
 
 
+def _run_open_shell_direct(kind: str, mol, *, basis: Any | None = None, eps_schwarz: float = 1e-12,
+                           direct_threads: int = 256, direct_max_tile_bytes: int = 256 << 20,
+                           direct_gpu_task_budget_bytes: int | None = None, direct_max_slab_tasks: int | None = None,
+                           max_cycle: int = 50, conv_tol: float = 1e-10, conv_tol_dm: float = 1e-8, diis: bool = True,
+                           diis_start_cycle: int = 2, diis_space: int = 8, damping: float = 0.0, dm0=None,
+                           mo_coeff0=None, mixed_precision: bool | None = None,
+                           profile: dict | None = None) -> RHFDFRunResult:
+    nelec = int(mol.nelectron)
+    spin = int(getattr(mol, "spin", 0))
+    if nelec <= 0 or (nelec + spin) % 2 != 0 or spin < 0 or spin > nelec:
+        raise ValueError("nelectron and spin (2S) must give nalpha = (nelec + spin)/2 >= nbeta >= 0")
+    if not bool(getattr(mol, "cart", True)):
+        raise NotImplementedError("Integral-direct SCF does not yet support spherical AOs (cart=False)")
+    nalpha, nbeta = (nelec + spin) // 2, (nelec - spin) // 2
+    ao_basis, basis_name = build_ao_basis_cart(mol, basis)
+    jk_ctx = make_direct_jk_context(ao_basis, eps_schwarz=float(eps_schwarz), threads=int(direct_threads),
+                                    max_tile_bytes=int(direct_max_tile_bytes),
+                                    max_slab_tasks=direct_max_slab_tasks,
+                                    gpu_task_budget_bytes=direct_gpu_task_budget_bytes,
+                                    atom_coords_bohr=mol.coords_bohr, atom_charges=mol.charges)
+    int1e = build_int1e(jk_ctx)
+    drv = uhf_direct if kind == "uhf" else rohf_direct
+    scf = drv(int1e.S, int1e.hcore, jk_ctx, nalpha=nalpha, nbeta=nbeta, enuc=float(mol.energy_nuc()),
+              max_cycle=int(max_cycle), conv_tol=float(conv_tol), conv_tol_dm=float(conv_tol_dm), diis=bool(diis),
+              diis_start_cycle=int(diis_start_cycle), diis_space=int(diis_space), damping=float(damping), dm0=dm0,
+              mo_coeff0=mo_coeff0, mixed_precision=mixed_precision, profile=profile)
+    return RHFDFRunResult(mol=mol, basis_name=str(basis_name), auxbasis_name="<direct>", ao_basis=ao_basis,
+                          aux_basis=None, int1e=int1e, df_B=None, scf=scf, profile=profile, two_e_backend="direct",
+                          direct_jk_ctx=jk_ctx)
+
+
+def run_uhf_direct(mol, **kwargs) -> RHFDFRunResult:
+    """UHF with integral-direct four-centre J/K (asuka/frontend/scf.py:502-560)."""
+    return _run_open_shell_direct("uhf", mol, **kwargs)
+
+
+def run_rohf_direct(mol, **kwargs) -> RHFDFRunResult:
+    """ROHF with integral-direct four-centre J/K (asuka/frontend/scf.py:549-)."""
+    return _run_open_shell_direct("rohf", mol, **kwargs)
+
+
+# the open-shell results carry the same fields (asuka/frontend/scf.py UHFDFRunResult, ROHFDFRunResult)
+UHFDFRunResult = RHFDFRunResult
+ROHFDFRunResult = RHFDFRunResult
+
+
 def run_hf_df(mol, *, method: str = "rhf", backend: str = "cuda", df: bool = True,
               two_e_backend: str | None = None, guess: Any | None = None, dm0=None, mo_coeff0=None, **kwargs):
-    """Unified HF entry point; this engine implements the integral-direct RHF
-    branch (asuka/frontend/_scf_dispatch.py:147-184)."""
-    if str(method).strip().lower() != "rhf":
-        raise NotImplementedError("only method='rhf' is implemented by the direct engine")
+    """Unified HF entry point; this engine implements the integral-direct RHF,
+    UHF and ROHF branch (asuka/frontend/_scf_dispatch.py:147-184)."""
+    method_s = str(method).strip().lower()
+    if method_s not in ("rhf", "uhf", "rohf"):
+        raise NotImplementedError("only method='rhf', 'uhf' or 'rohf' is implemented by the direct engine")
     backend_s = str(backend).strip().lower()
     if backend_s not in ("cuda", "hip", "gpu"):
         raise ValueError("two_e_backend='direct' requires backend='cuda' (the GPU)")
@@ -122,7 +171,12 @@ def run_hf_df(mol, *, method: str = "rhf", backend: str = "cuda", df: bool = Tru
             dm0 = guess
     for k in ("auxbasis", "df_layout", "df_threads"):
         kwargs.pop(k, None)
+    if method_s == "uhf":
+        return run_uhf_direct(mol, dm0=dm0, mo_coeff0=mo_coeff0, **kwargs)
+    if method_s == "rohf":
+        return run_rohf_direct(mol, dm0=dm0, mo_coeff0=mo_coeff0, **kwargs)
     return run_rhf_direct(mol, dm0=dm0, mo_coeff0=mo_coeff0, **kwargs)
 
 
-__all__ = ["Int1eResult", "RHFDFRunResult", "build_int1e", "run_hf_df", "run_rhf_direct"]
+__all__ = ["Int1eResult", "RHFDFRunResult", "ROHFDFRunResult", "UHFDFRunResult", "build_int1e", "run_hf_df",
+           "run_rhf_direct", "run_rohf_direct", "run_uhf_direct"]

@@ -27,7 +27,7 @@ from typing import Any
 
 import numpy as np
 
-from .direct_jk import DirectJKContext, direct_fock_rhf, direct_JK
+from .direct_jk import DirectJKContext, direct_fock_rhf, direct_JK, direct_JK_multi
 
 _CPU_EIGH_MAX_NAO = int(os.environ.get("ASUKA_HF_CPU_EIGH_MAX_NAO", "96") or 96)
 
@@ -256,4 +256,161 @@ def rhf_direct(
                      e_nuc=float(enuc), mo_energy=eps, mo_coeff=C, mo_occ=occ, dm=D)
 
 
-__all__ = ["SCFResult", "rhf_direct"]
+def _occ_uhf(nalpha: int, nbeta: int, nao: int):
+    """asuka/hf/df_scf.py:473-484."""
+    nalpha, nbeta = int(nalpha), int(nbeta)
+    if nalpha < 0 or nbeta < 0:
+        raise ValueError("nalpha/nbeta must be >= 0")
+    if nalpha > nao or nbeta > nao:
+        raise ValueError("nalpha/nbeta exceeds number of orbitals")
+    return nalpha, nbeta
+
+
+def _density_spin(C, nocc: int):
+    Co = C[:, :nocc]
+    return _sym(Co @ Co.T)
+
+
+def _roothaan_fock(torch, Fa, Fb, Da, Db, S):
+    """ROHF effective Fock matrix (asuka/hf/df_scf.py:691-719): with the core,
+    open and virtual projectors pc = Db S, po = (Da - Db) S, pv = 1 - Da S and
+    fc = (Fa + Fb)/2, F = G + G^T where G = 1/2 (pc^T fc pc + po^T fc po +
+    pv^T fc pv) + po^T Fb pc + po^T Fa pv + pv^T fc pc."""
+    nao = int(S.shape[0])
+    fc = 0.5 * (Fa + Fb)
+    pc = Db @ S
+    po = (Da - Db) @ S
+    pv = torch.eye(nao, dtype=torch.float64, device=S.device) - Da @ S
+    G = 0.5 * (pc.T @ fc @ pc + po.T @ fc @ po + pv.T @ fc @ pv)
+    G = G + po.T @ Fb @ pc + po.T @ Fa @ pv + pv.T @ fc @ pc
+    return G + G.T
+
+
+def _open_shell_scf(kind: str, S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
+                    diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile):
+    import torch
+
+    dev = jk_ctx.device
+    S = torch.as_tensor(np.asarray(S) if not isinstance(S, torch.Tensor) else S).to(dev, torch.float64)
+    h = torch.as_tensor(np.asarray(hcore) if not isinstance(hcore, torch.Tensor) else hcore).to(dev, torch.float64)
+    nao = int(S.shape[0])
+    if tuple(S.shape) != (nao, nao) or tuple(h.shape) != (nao, nao):
+        raise ValueError("S/hcore must be (nao, nao)")
+    if nao != jk_ctx.nao:
+        raise ValueError(f"nao mismatch: S has {nao}, DirectJKContext has {jk_ctx.nao}")
+    na, nb = _occ_uhf(nalpha, nbeta, nao)
+    if kind == "ROHF" and na < nb:
+        raise ValueError("ROHF requires nalpha >= nbeta")
+    occ_a = torch.zeros(nao, dtype=torch.float64, device=dev)
+    occ_b = torch.zeros(nao, dtype=torch.float64, device=dev)
+    occ_a[:na] = 1.0
+    occ_b[:nb] = 1.0
+    X = _orthogonalizer(torch, S)
+    e0, C = _gen_eigh(torch, h, X)
+    Ca = Cb = C
+    ea = eb = e0
+
+    def as_dev(A):
+        return torch.as_tensor(np.asarray(A) if not isinstance(A, torch.Tensor) else A).to(dev, torch.float64)
+
+    if dm0 is not None:
+        if not isinstance(dm0, (tuple, list)) or len(dm0) != 2:
+            raise TypeError(f"dm0 for {kind} must be a (Da, Db) tuple")
+        Da, Db = _sym(as_dev(dm0[0])), _sym(as_dev(dm0[1]))
+    else:
+        if mo_coeff0 is not None:
+            if isinstance(mo_coeff0, (tuple, list)):
+                if kind == "ROHF" or len(mo_coeff0) != 2:
+                    raise TypeError(f"mo_coeff0 for {kind} must be " + ("one matrix" if kind == "ROHF"
+                                                                          else "a (Ca, Cb) tuple"))
+                Ca, Cb = as_dev(mo_coeff0[0]), as_dev(mo_coeff0[1])
+            else:
+                Ca = Cb = as_dev(mo_coeff0)
+        Da, Db = _density_spin(Ca, na), _density_spin(Cb, nb)
+
+    lam = float(damping) if damping else 0.0
+    nd = 2 if kind == "UHF" else 1
+    diis_objs = [(_DIIS(torch, diis_space) if diis else None) for _ in range(nd)]
+    e_last = None
+    converged = False
+    cycle = 0
+    njk = 0
+    for cycle in range(1, int(max_cycle) + 1):
+        Ja, Ka, Jb, Kb = direct_JK_multi(jk_ctx, Da, Db, mixed_precision=mixed_precision)
+        njk += 1
+        J = Ja + Jb
+        Fa = _sym(h + J - Ka)
+        Fb = _sym(h + J - Kb)
+        if kind == "UHF":
+            Fs, Ds = [Fa, Fb], [Da, Db]
+        else:
+            Fs, Ds = [_roothaan_fock(torch, Fa, Fb, Da, Db, S)], [Da + Db]
+        for k in range(nd):
+            if diis_objs[k] is None or cycle < int(diis_start_cycle):
+                continue
+            M = Fs[k] @ Ds[k] @ S
+            diis_objs[k].push(Fs[k], M - M.T)
+            try:
+                Ft = diis_objs[k].extrapolate()
+            except Exception:
+                Ft = None
+            if Ft is None or not bool(torch.isfinite(Ft).all()):
+                diis_objs[k] = _DIIS(torch, diis_space)
+            else:
+                Fs[k] = _sym(Ft)
+        if kind == "UHF":
+            ea, Ca = _gen_eigh(torch, Fs[0], X)
+            eb, Cb = _gen_eigh(torch, Fs[1], X)
+        else:
+            ea, Ca = _gen_eigh(torch, Fs[0], X)
+            eb, Cb = ea, Ca
+        Da_n, Db_n = _density_spin(Ca, na), _density_spin(Cb, nb)
+        if lam:
+            Da_n = (1.0 - lam) * Da_n + lam * Da
+            Db_n = (1.0 - lam) * Db_n + lam * Db
+        Dt = Da_n + Db_n
+        # the reference's energy expression (asuka/hf/direct_scf.py:543-548):
+        # new densities with this cycle's J and K
+        e_elec = float((Dt * h).sum()) + 0.5 * float((Dt * J).sum()) \
+            - 0.5 * float((Da_n * Ka).sum()) - 0.5 * float((Db_n * Kb).sum())
+        e_tot = e_elec + float(enuc)
+        dm_err = float(torch.linalg.norm(Da_n - Da)) + float(torch.linalg.norm(Db_n - Db))
+        de = abs(e_tot - e_last) if e_last is not None else float("inf")
+        Da, Db, e_last = Da_n, Db_n, e_tot
+        if de < conv_tol and dm_err < conv_tol_dm:
+            converged = True
+            break
+    if profile is not None:
+        profile.setdefault("scf", {})["iters"] = int(cycle)
+        profile["scf"]["jk_calls"] = njk
+    mo_e = (ea, eb) if kind == "UHF" else ea
+    mo_c = (Ca, Cb) if kind == "UHF" else Ca
+    return SCFResult(method=kind, converged=converged, niter=int(cycle),
+                     e_tot=float(e_last if e_last is not None else float("nan")),
+                     e_elec=float(e_last - float(enuc)) if e_last is not None else float("nan"),
+                     e_nuc=float(enuc), mo_energy=mo_e, mo_coeff=mo_c, mo_occ=(occ_a, occ_b), dm=(Da, Db))
+
+
+def uhf_direct(S, hcore, jk_ctx: DirectJKContext, *, nalpha: int, nbeta: int, enuc: float = 0.0,
+               max_cycle: int = 50, conv_tol: float = 1e-10, conv_tol_dm: float = 1e-8, diis: bool = True,
+               diis_start_cycle: int = 2, diis_space: int = 8, damping: float = 0.0, dm0=None, mo_coeff0=None,
+               mixed_precision: bool | None = None, profile: dict | None = None) -> SCFResult:
+    """UHF with the two-density direct J/K build (asuka/hf/direct_scf.py:405-573):
+    core guess, per-spin Fock h + J_a + J_b - K_s, one DIIS per spin on
+    F D S - (F D S)^T, per-spin eigh, D_s = C_s,occ C_s,occ^T."""
+    return _open_shell_scf("UHF", S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
+                           diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile)
+
+
+def rohf_direct(S, hcore, jk_ctx: DirectJKContext, *, nalpha: int, nbeta: int, enuc: float = 0.0,
+                max_cycle: int = 50, conv_tol: float = 1e-10, conv_tol_dm: float = 1e-8, diis: bool = True,
+                diis_start_cycle: int = 2, diis_space: int = 8, damping: float = 0.0, dm0=None, mo_coeff0=None,
+                mixed_precision: bool | None = None, profile: dict | None = None) -> SCFResult:
+    """ROHF with the two-density direct J/K build (asuka/hf/direct_scf.py:576-):
+    Roothaan's effective Fock matrix from the spin Fock matrices, one DIIS on
+    F D_tot S - (F D_tot S)^T, one eigh, shared orbitals for both spins."""
+    return _open_shell_scf("ROHF", S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
+                           diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile)
+
+
+__all__ = ["SCFResult", "rhf_direct", "rohf_direct", "uhf_direct"]

@@ -379,6 +379,7 @@ def main() -> int:
     ap.add_argument("--fd", action="store_true", help="gradient workload: central-difference check on 6 coordinates")
     ap.add_argument("--autotune", action="store_true", help="J/K workloads: autotune launch shapes before timing")
     ap.add_argument("--scf", action="store_true", help="config 5: time 15 SCF iterations, FP64 and mixed")
+    ap.add_argument("--tau", type=float, default=1e-7, help="FP32 cut on Q_ab Q_cd max|D| of the mixed build")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (the calibration kernel of the HBM counters)")
     args = ap.parse_args()
@@ -421,7 +422,7 @@ def main() -> int:
     C, _ = np.linalg.qr(C)
     D_h = C @ C.T
     t0 = time.perf_counter()
-    ctx = make_direct_jk_context(ao, eps_schwarz=cfg["eps"])
+    ctx = make_direct_jk_context(ao, eps_schwarz=cfg["eps"], tau_mixed=args.tau)
     t_ctx = time.perf_counter() - t0
     log(f"context: nao={nao} nsp={ctx.nsp} schwarz-screened quartets={ctx.ntasks} ({t_ctx:.2f}s)")
 
@@ -491,7 +492,7 @@ def main() -> int:
         dK = res["mixed"]["K"] - res["fp64"]["K"]
         dE = float(0.5 * (D * dJ).sum() - 0.25 * (D * dK).sum())
         accuracy = {"max_abs_dJ": float(dJ.abs().max()), "max_abs_dK": float(dK.abs().max()),
-                    "dE_2e_Eh": dE, "tau_mixed": 1e-7,
+                    "dE_2e_Eh": dE, "tau_mixed": args.tau,
                     "fp64_s_per_build": res["fp64"]["s_per_build"],
                     "mixed_s_per_build": res["mixed"]["s_per_build"],
                     "speedup_mixed_vs_fp64": res["fp64"]["s_per_build"] / res["mixed"]["s_per_build"],

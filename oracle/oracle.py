@@ -449,8 +449,70 @@ def rhf(S, hcore, fock_fn, nelec: int, enuc: float = 0.0, max_cycle: int = 50, c
     return {"e_tot": e_last, "converged": conv, "niter": cycle, "mo_energy": eps, "mo_coeff": C, "dm": D}
 
 
+def open_shell_scf(S, hcore, jk_fn, nalpha: int, nbeta: int, enuc: float = 0.0, rohf: bool = False,
+                   max_cycle: int = 100, conv_tol: float = 1e-12, conv_tol_dm: float = 1e-10, diis_space: int = 8,
+                   diis_start_cycle: int = 2):
+    """UHF (hf/direct_scf.py:405-573) or ROHF (:576-) with jk_fn(D) -> (J, K):
+    core guess, spin Fock matrices Fs = h + J(Da + Db) - K(Ds), ROHF through
+    Roothaan's effective Fock matrix (hf/df_scf.py:691-719), DIIS on
+    F D S - (F D S)^T (per spin for UHF, on D_tot for ROHF), and the
+    reference's energy expression with this cycle's J and K and the new
+    densities (hf/direct_scf.py:543-548)."""
+    sym = lambda A: 0.5 * (A + A.T)
+    s, U = np.linalg.eigh(sym(S))
+    X = U @ np.diag(s ** -0.5) @ U.T
+    nao = S.shape[0]
+
+    def gen_eigh(F):
+        e, Cp = np.linalg.eigh(sym(X.T @ sym(F) @ X))
+        return e, X @ Cp
+
+    def dens(C, n):
+        return sym(C[:, :n] @ C[:, :n].T)
+
+    e0, C = gen_eigh(hcore)
+    Da, Db = dens(C, nalpha), dens(C, nbeta)
+    nd = 1 if rohf else 2
+    diis = [_DIIS(diis_space) for _ in range(nd)]
+    e_last, conv, cycle = None, False, 0
+    for cycle in range(1, max_cycle + 1):
+        J, Ka = jk_fn(Da + Db)[0], jk_fn(Da)[1]
+        Kb = jk_fn(Db)[1]
+        Fa, Fb = sym(hcore + J - Ka), sym(hcore + J - Kb)
+        if rohf:
+            fc = 0.5 * (Fa + Fb)
+            pc, po, pv = Db @ S, (Da - Db) @ S, np.eye(nao) - Da @ S
+            G = 0.5 * (pc.T @ fc @ pc + po.T @ fc @ po + pv.T @ fc @ pv) + po.T @ Fb @ pc + po.T @ Fa @ pv \
+                + pv.T @ fc @ pc
+            Fs, Ds = [G + G.T], [Da + Db]
+        else:
+            Fs, Ds = [Fa, Fb], [Da, Db]
+        if cycle >= diis_start_cycle:
+            for k in range(nd):
+                M = Fs[k] @ Ds[k] @ S
+                diis[k].push(Fs[k], M - M.T)
+                Ft = diis[k].extrapolate()
+                if Ft is None or not np.all(np.isfinite(Ft)):
+                    diis[k] = _DIIS(diis_space)
+                else:
+                    Fs[k] = sym(Ft)
+        ea, Ca = gen_eigh(Fs[0])
+        eb, Cb = (ea, Ca) if rohf else gen_eigh(Fs[1])
+        Dan, Dbn = dens(Ca, nalpha), dens(Cb, nbeta)
+        Dt = Dan + Dbn
+        e_tot = float(np.sum(Dt * hcore) + 0.5 * np.sum(Dt * J) - 0.5 * np.sum(Dan * Ka) - 0.5 * np.sum(Dbn * Kb)) \
+            + enuc
+        dm_err = float(np.linalg.norm(Dan - Da) + np.linalg.norm(Dbn - Db))
+        de = abs(e_tot - e_last) if e_last is not None else float("inf")
+        Da, Db, e_last = Dan, Dbn, e_tot
+        if de < conv_tol and dm_err < conv_tol_dm:
+            conv = True
+            break
+    return {"e_tot": e_last, "converged": conv, "niter": cycle, "dm": (Da, Db)}
+
+
 __all__ = [
     "boys_list", "build", "dense_JK", "dense_eri", "density_screen", "direct_jk", "direct_jk_tasks", "eri_tile",
-    "int1e", "lib", "make_context", "rhf", "rys_roots_weights", "schwarz", "shell_pair_dmax", "shell_pairs_l_order",
-    "tasks_screened_sorted_q",
+    "int1e", "lib", "make_context", "open_shell_scf", "rhf", "rys_roots_weights", "schwarz", "shell_pair_dmax",
+    "shell_pairs_l_order", "tasks_screened_sorted_q",
 ]
