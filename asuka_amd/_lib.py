@@ -83,6 +83,11 @@ SYMBOLS = [
     ("asuka_int1e", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("asuka_jk_build_batched", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    ("asuka_jk_batch_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.POINTER(BasisDesc)),
+                                             ctypes.POINTER(JKOpts), _vp, ctypes.POINTER(_vp)]),
+    ("asuka_jk_batch_info", ctypes.c_int, [_vp, _i32p, ctypes.c_int]),
+    ("asuka_jk_batch_build", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            ctypes.c_int, _vp]),
     ("asuka_jk_grad", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("asuka_int1e_grad", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("asuka_rys_roots", ctypes.c_int, [ctypes.c_int, ctypes.c_double, _f64p, _f64p]),

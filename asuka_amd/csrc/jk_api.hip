@@ -144,24 +144,37 @@ __global__ void k_symmetrize(double* X, int n) {
 
 // max |D| over each shell-pair block, both orientations (asuka/hf/direct_jk.py:1484-1527);
 // with a second density D2 (two-density builds) the max over both.
-__global__ void k_dsp(const double* D, const double* D2, const int* sp_A, const int* sp_B, const int* shell_ao,
-                      const int* shell_l, int nsp, int nao, double* Dsp) {
+__global__ void k_dsp(const double* D, const double* D2, DevBasis b, double* Dsp) {
   const int sp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sp >= nsp) return;
-  const int A = sp_A[sp], B = sp_B[sp];
-  const int a0 = shell_ao[A], b0 = shell_ao[B];
-  const int nA = ncart_c(shell_l[A]), nB = ncart_c(shell_l[B]);
+  if (sp >= b.nsp) return;
+  const int A = b.sp_A[sp], B = b.sp_B[sp];
+  long long N, msh;
+  mol_layout(b, A, N, msh);
+  const int a0 = b.shell_ao[A], b0 = b.shell_ao[B];
+  const int nA = ncart_c(b.shell_l[A]), nB = ncart_c(b.shell_l[B]);
+  const double* X = D + msh;
+  const double* X2 = D2 ? D2 + msh : nullptr;
   double m = 0.0;
   for (int i = 0; i < nA; ++i)
     for (int j = 0; j < nB; ++j) {
-      m = fmax(m, fabs(D[(long long)(a0 + i) * nao + b0 + j]));
-      if (A != B) m = fmax(m, fabs(D[(long long)(b0 + j) * nao + a0 + i]));
-      if (D2) {
-        m = fmax(m, fabs(D2[(long long)(a0 + i) * nao + b0 + j]));
-        if (A != B) m = fmax(m, fabs(D2[(long long)(b0 + j) * nao + a0 + i]));
+      m = fmax(m, fabs(X[(a0 + i) * N + b0 + j]));
+      if (A != B) m = fmax(m, fabs(X[(b0 + j) * N + a0 + i]));
+      if (X2) {
+        m = fmax(m, fabs(X2[(a0 + i) * N + b0 + j]));
+        if (A != B) m = fmax(m, fabs(X2[(b0 + j) * N + a0 + i]));
       }
     }
   Dsp[sp] = m;
+}
+
+// out = (X + X^T) / 2 for an n x n block X (batch builds: the packed block of
+// one molecule to the caller's matrix).
+__global__ void k_sym_copy(double* out, const double* X, int n) {
+  const long long nn = (long long)n * n;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nn; t += (long long)gridDim.x * blockDim.x) {
+    const long long i = t / n, j = t - i * n;
+    out[t] = 0.5 * (X[t] + X[j * n + i]);
+  }
 }
 
 __global__ void k_rys_points(int n, long long npts, const double* T, double* x, double* w) {
@@ -205,6 +218,11 @@ struct asuka_jk_ctx {
   double eps_schwarz = 0, eps_density = 0, tau_mixed = 0;
   std::vector<int> shell_l, shell_ao, shell_atom, sp_A, sp_B, pp_start, pp_count;
   std::vector<int> pp_count0;  // primitive pairs per shell pair before primitive-pair screening
+  std::vector<int> shell_mol;  // molecule of each shell (batch contexts; all 0 otherwise)
+  int nmol = 1;
+  std::vector<int> mol_ao_off, mol_nao;  // each molecule's first AO and AO count
+  long long nelem = 0;                   // elements of a D/J/K buffer (sum of the molecules' n^2)
+  double *d_Dbig = nullptr, *d_Jbig = nullptr, *d_Kbig = nullptr;  // batch contexts: block-diagonal D, J, K
   std::vector<double> shell_xyz, Q, atom_xyz, atom_Z;
   std::vector<PrimPair> pp;
   std::vector<PrimPairF> ppf;
@@ -289,10 +307,15 @@ int asuka_jk_version(void) { return ASUKA_JK_ABI_VERSION; }
 
 const char* asuka_jk_last_error(void) { return g_err.c_str(); }
 
-int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, void* stream, asuka_jk_ctx** out) {
+}  // extern "C"
+
+// Builds a context for the shells of bd. shell_mol (optional) assigns each
+// shell to one of several independent molecules: shell pairs, quartets and
+// the LDS accumulator columns then stay within a molecule (a batch context).
+static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_mol, const asuka_jk_opts* opts,
+                     hipStream_t s, asuka_jk_ctx** out) {
   if (!bd || !opts || !out) return fail(ASUKA_E_INVALID, "basis, opts and out must be non-null");
   *out = nullptr;
-  hipStream_t s = (hipStream_t)stream;
   const int nshell = bd->nshell;
   if (nshell <= 0) return fail(ASUKA_E_INVALID, "AO basis has zero shells");
   if (!bd->shell_cxyz || !bd->shell_prim_start || !bd->shell_nprim || !bd->shell_l || !bd->shell_ao_start ||
@@ -322,6 +345,14 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
     if (np <= 0 || ps < 0 || ps + np > bd->nprim) return fail(ASUKA_E_INVALID, "shell primitive range out of bounds");
   }
   ctx->nao = ctx->shell_ao[nshell - 1] + ncart_c(ctx->shell_l[nshell - 1]);
+  if (shell_mol) {
+    if ((int)shell_mol->size() != nshell) return fail(ASUKA_E_INVALID, "shell_mol must have nshell entries");
+    ctx->shell_mol = *shell_mol;
+  } else {
+    ctx->shell_mol.assign(nshell, 0);
+  }
+  ctx->nmol = 1 + *std::max_element(ctx->shell_mol.begin(), ctx->shell_mol.end());
+  const int nmol = ctx->nmol;
   ctx->natm = bd->natm;
   if (bd->natm > 0) {
     if (!bd->atom_xyz || !bd->atom_charge) return fail(ASUKA_E_INVALID, "atom arrays must be non-null when natm > 0");
@@ -351,6 +382,7 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   // ---- shell pairs, l-ordered orientation (asuka/cueri/shell_pairs.py:72-116) ----
   for (int i = 0; i < nshell; ++i)
     for (int j = 0; j <= i; ++j) {
+      if (ctx->shell_mol[i] != ctx->shell_mol[j]) continue;  // no pairs across the molecules of a batch
       int A = i, B = j;
       if (ctx->shell_l[i] < ctx->shell_l[j]) std::swap(A, B);
       ctx->sp_A.push_back(A);
@@ -423,24 +455,65 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
     if ((rc = dev_upload(&ctx->d_prim_exp, pe, own, s))) return rc;
     if ((rc = dev_upload(&ctx->d_prim_coef, pc, own, s))) return rc;
   }
-  // compact AO numbering per angular momentum (the LDS accumulator columns)
+  // compact AO numbering per angular momentum and molecule (the LDS
+  // accumulator columns): lao_ao lists, for each l, the AOs of angular
+  // momentum l molecule by molecule; shell_lao is a shell's first index in
+  // its molecule's list and mol_lao where that list starts within l's part.
   {
-    std::vector<int> shell_lao(nshell), lao_ao;
+    std::vector<int> shell_lao(nshell), lao_ao, mol_lao((size_t)nmol * (ASUKA_LMAX + 1));
     for (int l = 0; l <= ASUKA_LMAX; ++l) {
       ctx->lao_off[l] = (int)lao_ao.size();
-      for (int sh = 0; sh < nshell; ++sh)
-        if (ctx->shell_l[sh] == l) {
-          shell_lao[sh] = (int)lao_ao.size() - ctx->lao_off[l];
-          for (int c = 0; c < ncart_c(l); ++c) lao_ao.push_back(ctx->shell_ao[sh] + c);
-        }
-      ctx->nlao[l] = (int)lao_ao.size() - ctx->lao_off[l];
+      ctx->nlao[l] = 0;
+      for (int m = 0; m < nmol; ++m) {
+        const int m0 = (int)lao_ao.size();
+        mol_lao[(size_t)m * (ASUKA_LMAX + 1) + l] = m0 - ctx->lao_off[l];
+        for (int sh = 0; sh < nshell; ++sh)
+          if (ctx->shell_l[sh] == l && ctx->shell_mol[sh] == m) {
+            shell_lao[sh] = (int)lao_ao.size() - m0;
+            for (int c = 0; c < ncart_c(l); ++c) lao_ao.push_back(ctx->shell_ao[sh] + c);
+          }
+        ctx->nlao[l] = std::max(ctx->nlao[l], (int)lao_ao.size() - m0);  // columns of the largest molecule
+      }
     }
     ctx->lao_off[ASUKA_LMAX + 1] = (int)lao_ao.size();
-    int *d_slao, *d_laoao;
+    int *d_slao, *d_laoao, *d_mlao, *d_smol;
     if ((rc = dev_upload(&d_slao, shell_lao, own, s))) return rc;
     if ((rc = dev_upload(&d_laoao, lao_ao, own, s))) return rc;
+    if ((rc = dev_upload(&d_mlao, mol_lao, own, s))) return rc;
+    if ((rc = dev_upload(&d_smol, ctx->shell_mol, own, s))) return rc;
     ctx->dev.shell_lao = d_slao;
     ctx->dev.lao_ao = d_laoao;
+    ctx->dev.mol_lao = d_mlao;
+    ctx->dev.shell_mol = d_smol;
+  }
+  // per-molecule D/J/K blocks (a molecule's AOs are contiguous): row-major
+  // n_m x n_m blocks packed one after another; one molecule: plain nao x nao
+  {
+    std::vector<int> lo(nmol, ctx->nao), hi(nmol, 0), cnt(nmol, 0);
+    for (int sh = 0; sh < nshell; ++sh) {
+      const int m = ctx->shell_mol[sh], n = ncart_c(ctx->shell_l[sh]);
+      lo[m] = std::min(lo[m], ctx->shell_ao[sh]);
+      hi[m] = std::max(hi[m], ctx->shell_ao[sh] + n);
+      cnt[m] += n;
+    }
+    std::vector<int> mn(nmol);
+    std::vector<long long> msh(nmol);
+    long long base = 0;
+    for (int m = 0; m < nmol; ++m) {
+      if (cnt[m] == 0 || hi[m] - lo[m] != cnt[m]) return fail(ASUKA_E_INVALID, "each molecule's AOs must be contiguous");
+      mn[m] = cnt[m];
+      msh[m] = base - (long long)lo[m] * (cnt[m] + 1);
+      base += (long long)cnt[m] * cnt[m];
+    }
+    ctx->mol_ao_off = lo;
+    ctx->mol_nao = mn;
+    ctx->nelem = base;
+    int* d_mn;
+    long long* d_msh;
+    if ((rc = dev_upload(&d_mn, mn, own, s))) return rc;
+    if ((rc = dev_upload(&d_msh, msh, own, s))) return rc;
+    ctx->dev.mol_n = d_mn;
+    ctx->dev.mol_shift = d_msh;
   }
   std::vector<double> zeros(ctx->nsp, 0.0);
   if ((rc = dev_upload(&ctx->d_Q, zeros, own, s))) return rc;
@@ -575,11 +648,14 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   }
 
   // ---- class lists: pairs of each class, Q descending (stable), Q > 0 ----
+  // (per molecule of a batch: cls_pairs[m * ncls + class])
   const int ncls = pair_class(ASUKA_LMAX, ASUKA_LMAX) + 1;
-  std::vector<std::vector<int>> cls_pairs(ncls);
+  std::vector<std::vector<int>> cls_pairs((size_t)nmol * ncls);
   for (int sp = 0; sp < ctx->nsp; ++sp) {
     if (!(ctx->Q[sp] > 0.0)) continue;
-    cls_pairs[pair_class(ctx->shell_l[ctx->sp_A[sp]], ctx->shell_l[ctx->sp_B[sp]])].push_back(sp);
+    cls_pairs[(size_t)ctx->shell_mol[ctx->sp_A[sp]] * ncls +
+              pair_class(ctx->shell_l[ctx->sp_A[sp]], ctx->shell_l[ctx->sp_B[sp]])]
+        .push_back(sp);
   }
   for (auto& v : cls_pairs)
     std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return ctx->Q[a] > ctx->Q[b]; });
@@ -608,13 +684,13 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
     std::vector<long long> kpref;  // prefix sums of the pairs' primitive-pair counts
     int col_start = 0;
   };
-  std::vector<std::vector<PairGroup>> groups(ncls);
+  std::vector<std::vector<PairGroup>> groups((size_t)nmol * ncls);
   std::vector<int> col_all, row_pair_all;
   std::vector<long long> row_off_all;
   std::vector<Segment> segs;
   std::vector<WaveUnit> units;
   std::vector<RowTask> tasks;
-  for (int c = 0; c < ncls; ++c) {
+  for (size_t c = 0; c < cls_pairs.size(); ++c) {
     std::vector<int> kvals;
     for (int sp : cls_pairs[c]) kvals.push_back(kbucket(ctx->pp_count[sp]));
     std::sort(kvals.begin(), kvals.end());
@@ -661,8 +737,9 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
         cp.task_units = task_units;
       }
       long long padded = 0;
-      for (const PairGroup& bg : groups[X]) {
-        for (const PairGroup& kg : groups[Y]) {
+      for (int m = 0; m < nmol; ++m)
+      for (const PairGroup& bg : groups[(size_t)m * ncls + X]) {
+        for (const PairGroup& kg : groups[(size_t)m * ncls + Y]) {
           const auto& cols = kg.pairs;
           long long off = 0;
           std::vector<long long> offs;
@@ -755,6 +832,85 @@ int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, v
   HIP_CHECK(hipStreamSynchronize(s));
   *out = guard.release();
   return ASUKA_OK;
+}
+
+extern "C" {
+
+int asuka_jk_ctx_create(const asuka_basis_desc* bd, const asuka_jk_opts* opts, void* stream, asuka_jk_ctx** out) {
+  return ctx_build(bd, nullptr, opts, (hipStream_t)stream, out);
+}
+
+int asuka_jk_batch_create(int nmol, const asuka_basis_desc* const* bases, const asuka_jk_opts* opts, void* stream,
+                          asuka_jk_ctx** out) {
+  if (nmol < 1 || !bases || !opts || !out) return fail(ASUKA_E_INVALID, "nmol must be >= 1 and bases, opts, out non-null");
+  *out = nullptr;
+  // one basis holding every molecule's shells, AOs and primitives back to back
+  std::vector<double> cxyz, pexp, pcoef, axyz, aZ;
+  std::vector<int> pst, pnp, sl, sao, sat, smol;
+  int nao = 0, nprim = 0, natm = 0;
+  bool atoms = true;
+  for (int m = 0; m < nmol; ++m) {
+    const asuka_basis_desc* b = bases[m];
+    if (!b || b->nshell <= 0 || !b->shell_cxyz || !b->shell_prim_start || !b->shell_nprim || !b->shell_l ||
+        !b->shell_ao_start || !b->prim_exp || !b->prim_coef)
+      return fail(ASUKA_E_INVALID, "every basis must be non-null with non-null arrays and nshell > 0");
+    int mnao = 0;
+    for (int i = 0; i < b->nshell; ++i) {
+      if (b->shell_l[i] < 0 || b->shell_l[i] > ASUKA_LMAX)
+        return fail(b->shell_l[i] < 0 ? ASUKA_E_INVALID : ASUKA_E_UNSUPPORTED,
+                    "direct J/K supports only 0 <= l <= " + std::to_string(ASUKA_LMAX));
+      cxyz.insert(cxyz.end(), b->shell_cxyz + 3 * i, b->shell_cxyz + 3 * i + 3);
+      pst.push_back(nprim + b->shell_prim_start[i]);
+      pnp.push_back(b->shell_nprim[i]);
+      sl.push_back(b->shell_l[i]);
+      sao.push_back(nao + b->shell_ao_start[i]);
+      sat.push_back(natm + (b->shell_atom ? b->shell_atom[i] : 0));
+      smol.push_back(m);
+      mnao = std::max(mnao, b->shell_ao_start[i] + ncart_c(b->shell_l[i]));
+    }
+    pexp.insert(pexp.end(), b->prim_exp, b->prim_exp + b->nprim);
+    pcoef.insert(pcoef.end(), b->prim_coef, b->prim_coef + b->nprim);
+    if (b->natm > 0 && b->atom_xyz && b->atom_charge) {
+      axyz.insert(axyz.end(), b->atom_xyz, b->atom_xyz + 3 * b->natm);
+      aZ.insert(aZ.end(), b->atom_charge, b->atom_charge + b->natm);
+    } else {
+      atoms = false;
+    }
+    nao += mnao;
+    nprim += b->nprim;
+    natm += std::max(0, b->natm);
+  }
+  asuka_basis_desc c{};
+  c.nshell = (int32_t)sl.size();
+  c.shell_cxyz = cxyz.data();
+  c.shell_prim_start = pst.data();
+  c.shell_nprim = pnp.data();
+  c.shell_l = sl.data();
+  c.shell_ao_start = sao.data();
+  c.shell_atom = sat.data();
+  c.nprim = nprim;
+  c.prim_exp = pexp.data();
+  c.prim_coef = pcoef.data();
+  c.natm = atoms ? natm : 0;
+  c.atom_xyz = atoms ? axyz.data() : nullptr;
+  c.atom_charge = atoms ? aZ.data() : nullptr;
+  asuka_jk_ctx* ctx = nullptr;
+  int rc = ctx_build(&c, &smol, opts, (hipStream_t)stream, &ctx);
+  if (rc) return rc;
+  std::unique_ptr<asuka_jk_ctx> guard(ctx);
+  for (double** p : {&ctx->d_Dbig, &ctx->d_Jbig, &ctx->d_Kbig}) {
+    const hipError_t e = hipMalloc((void**)p, sizeof(double) * (size_t)ctx->nelem);
+    if (e != hipSuccess) return fail(ASUKA_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    ctx->owned.push_back(*p);
+  }
+  *out = guard.release();
+  return ASUKA_OK;
+}
+
+int asuka_jk_batch_info(const asuka_jk_ctx* ctx, int32_t* mol_nao, int cap) {
+  if (!ctx || (cap > 0 && !mol_nao)) return fail(ASUKA_E_INVALID, "ctx and mol_nao must be non-null");
+  for (int m = 0; m < std::min(cap, ctx->nmol); ++m) mol_nao[m] = ctx->mol_nao[m];
+  return ctx->nmol;
 }
 
 int asuka_jk_ctx_destroy(asuka_jk_ctx* ctx) {
@@ -900,14 +1056,13 @@ static inline unsigned grid_for(long long n, int block) {
 // J/K of one density (D2 == nullptr) or of two densities from one evaluation
 // of each quartet; the outputs are zeroed first and symmetrised at the end.
 static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, double* J, double* K, double* J2,
-                         double* K2, bool wantJ, bool wantK, int flags, hipStream_t s) {
-  const long long nn = (long long)ctx->nao * ctx->nao;
+                         double* K2, bool wantJ, bool wantK, int flags, hipStream_t s, bool symmetrize = true) {
+  const long long nn = ctx->nelem;
   double* outs[4] = {wantJ ? J : nullptr, wantK ? K : nullptr, (D2 && wantJ) ? J2 : nullptr,
                      (D2 && wantK) ? K2 : nullptr};
   for (double* X : outs)
     if (X) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, X, nn, 0.0);
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, D2, ctx->dev.sp_A, ctx->dev.sp_B,
-                     ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, D2, ctx->dev, ctx->d_Dsp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_JK;
@@ -929,6 +1084,7 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
   }
   int rc = run_classes(ctx, o, s, (flags & ASUKA_JK_TIME) != 0);
   if (rc) return rc;
+  if (!symmetrize) return ASUKA_OK;
   const int n = ctx->nao;
   dim3 blk(16, 16), grd((n + 15) / 16, (n + 15) / 16);
   for (double* X : outs)
@@ -937,10 +1093,48 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
   return ASUKA_OK;
 }
 
+static int reject_batch(const asuka_jk_ctx* ctx) {
+  return ctx->nmol > 1 ? fail(ASUKA_E_INVALID, "a batch context must be used with asuka_jk_batch_build") : ASUKA_OK;
+}
+
 extern "C" {
+
+int asuka_jk_batch_build(asuka_jk_ctx* ctx, const double* const* D, double* const* J, double* const* K, int flags,
+                         void* stream) {
+  if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  const int nmol = ctx->nmol;
+  for (int m = 0; m < nmol; ++m)
+    if (!D[m]) return fail(ASUKA_E_INVALID, "every D[m] must be non-null");
+  const bool wantJ = (flags & ASUKA_JK_WANT_J) && J;
+  const bool wantK = (flags & ASUKA_JK_WANT_K) && K;
+  if (!wantJ && !wantK) return ASUKA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (!ctx->d_Dbig) return fail(ASUKA_E_INVALID, "ctx is not a batch context (asuka_jk_batch_create)");
+  long long base = 0;
+  std::vector<long long> bases(nmol);
+  for (int m = 0; m < nmol; ++m) {  // the molecules' densities into their packed blocks
+    const long long nn = (long long)ctx->mol_nao[m] * ctx->mol_nao[m];
+    bases[m] = base;
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, ctx->d_Dbig + base, D[m], nn);
+    base += nn;
+  }
+  HIP_CHECK(hipGetLastError());
+  int rc = jk_build_impl(ctx, ctx->d_Dbig, nullptr, wantJ ? ctx->d_Jbig : nullptr, wantK ? ctx->d_Kbig : nullptr,
+                         nullptr, nullptr, wantJ, wantK, flags, s, false);
+  if (rc) return rc;
+  for (int m = 0; m < nmol; ++m) {  // symmetrised blocks to the caller's matrices
+    const int n = ctx->mol_nao[m];
+    const long long nn = (long long)n * n;
+    if (wantJ && J[m]) hipLaunchKernelGGL(k_sym_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, J[m], ctx->d_Jbig + bases[m], n);
+    if (wantK && K[m]) hipLaunchKernelGGL(k_sym_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, K[m], ctx->d_Kbig + bases[m], n);
+  }
+  HIP_CHECK(hipGetLastError());
+  return ASUKA_OK;
+}
 
 int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream) {
   if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
   const bool wantJ = (flags & ASUKA_JK_WANT_J) && J;
   const bool wantK = (flags & ASUKA_JK_WANT_K) && K;
   if (!wantJ && !wantK) return ASUKA_OK;
@@ -950,6 +1144,7 @@ int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int
 int asuka_jk_build_multi(asuka_jk_ctx* ctx, const double* Da, const double* Db, double* Ja, double* Ka, double* Jb,
                          double* Kb, int flags, void* stream) {
   if (!ctx || !Da || !Db) return fail(ASUKA_E_INVALID, "ctx, Da and Db must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
   const bool wantJ = (flags & ASUKA_JK_WANT_J) != 0;
   const bool wantK = (flags & ASUKA_JK_WANT_K) != 0;
   if (wantJ && (!Ja || !Jb)) return fail(ASUKA_E_INVALID, "Ja and Jb must be non-null with ASUKA_JK_WANT_J");
@@ -964,12 +1159,12 @@ int asuka_jk_build_multi(asuka_jk_ctx* ctx, const double* Da, const double* Db, 
 
 int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, double* F, int flags, void* stream) {
   if (!ctx || !D || !hcore || !F) return fail(ASUKA_E_INVALID, "ctx, D, hcore and F must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
   if (F == D) return fail(ASUKA_E_INVALID, "F must not alias D");
   hipStream_t s = (hipStream_t)stream;
   const long long nn = (long long)ctx->nao * ctx->nao;
   if (F != hcore) hipLaunchKernelGGL(k_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, F, hcore, nn);
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev.sp_A, ctx->dev.sp_B,
-                     ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev, ctx->d_Dsp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_FOCK;
@@ -1053,6 +1248,7 @@ extern "C" {
 
 int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, void* stream) {
   if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
   if (reps < 1) return fail(ASUKA_E_INVALID, "reps must be >= 1");
   hipStream_t s = (hipStream_t)stream;
   const long long nn = (long long)ctx->nao * ctx->nao;
@@ -1072,8 +1268,7 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
       (void)hipEventDestroy(b);
     }
   } free_ev{e0, e1};
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev.sp_A, ctx->dev.sp_B,
-                     ctx->dev.shell_ao, ctx->dev.shell_l, ctx->nsp, ctx->nao, ctx->d_Dsp);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev, ctx->d_Dsp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_JK;
@@ -1222,6 +1417,7 @@ int asuka_jk_eri_tiles(asuka_jk_ctx* ctx, int64_t nq, const int32_t* spab, const
 
 int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream) {
   if (!ctx) return fail(ASUKA_E_INVALID, "ctx must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
   if (V && ctx->natm <= 0) return fail(ASUKA_E_INVALID, "V needs atoms (natm > 0)");
   HIP_CHECK(launch_int1e(ctx, S, T, V, (hipStream_t)stream));
   return ASUKA_OK;
@@ -1230,6 +1426,7 @@ int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream
 int asuka_jk_grad(asuka_jk_ctx* ctx, const double* D, const double* WJ, const double* WK, double* grad,
                   void* stream) {
   if (!ctx || !D || !grad) return fail(ASUKA_E_INVALID, "ctx, D and grad must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
   if (ctx->natm <= 0) return fail(ASUKA_E_INVALID, "the gradient needs atoms (natm > 0)");
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, grad, (long long)3 * ctx->natm, 0.0);
@@ -1254,6 +1451,7 @@ int asuka_jk_grad(asuka_jk_ctx* ctx, const double* D, const double* WJ, const do
 
 int asuka_int1e_grad(asuka_jk_ctx* ctx, const double* Dh, const double* Ws, double* grad, void* stream) {
   if (!ctx || !grad) return fail(ASUKA_E_INVALID, "ctx and grad must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
   if (ctx->natm <= 0) return fail(ASUKA_E_INVALID, "the gradient needs atoms (natm > 0)");
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, grad, (long long)3 * ctx->natm, 0.0);

@@ -75,7 +75,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
   double* Dad = Dbc + NB * NC;
   double* Dac = Dad + NA * ND;
   const int lane = threadIdx.x;
-  const long long N = b.nao;
   constexpr int oA = LA * (LA + 1) * (LA + 2) / 6, oB = LB * (LB + 1) * (LB + 2) / 6;
   constexpr int oC = LC * (LC + 1) * (LC + 2) / 6, oD = LD * (LD + 1) * (LD + 2) / 6;
 
@@ -91,6 +90,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     const double AB[3] = {cA[0] - cB[0], cA[1] - cB[1], cA[2] - cB[2]};
     const int aoA = b.shell_ao[A], aoB = b.shell_ao[B];
     const int ab0 = b.pp_start[P], nab = b.pp_count[P];
+    long long N, msh;  // the molecule's D/J/K block
+    mol_layout(b, A, N, msh);
+    const double* Dm = o.D + msh;
     {
       const int Qp = w.col_pair[un.col + jk];
       if (o.eps_density > 0.0 && (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]) < o.eps_density) continue;
@@ -108,12 +110,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
       __syncthreads();  // the previous quartet's digestion is done with the LDS
       for (int e = lane; e < K::NCOMP; e += kWave) tile[e] = 0.0;
       // density blocks of this quartet
-      for (int e = lane; e < NCD; e += kWave) Dcd[e] = o.D[(aoC + e / ND) * N + aoD + e % ND];
-      for (int e = lane; e < NAB; e += kWave) Dab[e] = o.D[(aoA + e / NB) * N + aoB + e % NB];
-      for (int e = lane; e < NB * ND; e += kWave) Dbd[e] = o.D[(aoB + e / ND) * N + aoD + e % ND];
-      for (int e = lane; e < NB * NC; e += kWave) Dbc[e] = o.D[(aoB + e / NC) * N + aoC + e % NC];
-      for (int e = lane; e < NA * ND; e += kWave) Dad[e] = o.D[(aoA + e / ND) * N + aoD + e % ND];
-      for (int e = lane; e < NA * NC; e += kWave) Dac[e] = o.D[(aoA + e / NC) * N + aoC + e % NC];
+      for (int e = lane; e < NCD; e += kWave) Dcd[e] = Dm[(aoC + e / ND) * N + aoD + e % ND];
+      for (int e = lane; e < NAB; e += kWave) Dab[e] = Dm[(aoA + e / NB) * N + aoB + e % NB];
+      for (int e = lane; e < NB * ND; e += kWave) Dbd[e] = Dm[(aoB + e / ND) * N + aoD + e % ND];
+      for (int e = lane; e < NB * NC; e += kWave) Dbc[e] = Dm[(aoB + e / NC) * N + aoC + e % NC];
+      for (int e = lane; e < NA * ND; e += kWave) Dad[e] = Dm[(aoA + e / ND) * N + aoD + e % ND];
+      for (int e = lane; e < NA * NC; e += kWave) Dac[e] = Dm[(aoA + e / NC) * N + aoC + e % NC];
 
       for (int ipq = 0; ipq < nab * ncd; ++ipq) {
         const int iab_p = ipq / ncd, icd_p = ipq - iab_p * ncd;
@@ -213,11 +215,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
       const double fk = (bk ? 2.0 : 1.0) * (o.mode == MODE_FOCK ? -0.5 : 1.0);
       const int nden = o.D2 ? 2 : 1;
       for (int di = 0; di < nden; ++di) {
-        double* Jo = di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
-        double* Ko = di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+        double* Jo = shifted(di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
+        double* Ko = shifted(di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
         if (di) {  // the density blocks of D2 replace those of D
           __syncthreads();
-          const double* D2 = o.D2;
+          const double* D2 = o.D2 + msh;
           for (int e = lane; e < NCD; e += kWave) Dcd[e] = D2[(aoC + e / ND) * N + aoD + e % ND];
           for (int e = lane; e < NAB; e += kWave) Dab[e] = D2[(aoA + e / NB) * N + aoB + e % NB];
           for (int e = lane; e < NB * ND; e += kWave) Dbd[e] = D2[(aoB + e / ND) * N + aoD + e % ND];

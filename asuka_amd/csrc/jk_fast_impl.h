@@ -370,7 +370,8 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   // workgroup's LDS accumulators when present (flushed once per row task),
   // J[c,d] (distinct per lane) to global atomics. With a second density
   // (two-density builds) the same tile is digested again into (J2, K2).
-  const long long N = b.nao;
+  long long N, msh;  // the molecule's D/J/K block (uniform: the bra pair is)
+  mol_layout(b, A, N, msh);
   const int aoA = b.shell_ao[A] + a0, aoB = b.shell_ao[B], aoC = b.shell_ao[C] + c0, aoD = b.shell_ao[Dsh];
   const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
   const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
@@ -381,9 +382,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   const int colC = lds ? b.shell_lao[C] + c0 : 0, colD = lds ? b.shell_lao[Dsh] : 0;
   const int nden = o.D2 ? 2 : 1;  // uniform
   for (int di = 0; di < nden; ++di) {
-    const double* Dm = di ? o.D2 : o.D;
-    double* Jo = di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
-    double* Ko = di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+    const double* Dm = (di ? o.D2 : o.D) + msh;
+    double* Jo = shifted(di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
+    double* Ko = shifted(di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
     const int so = di * acc.stride;  // this density's accumulators in s_acc
     if (Jo) {
       double dcd[NCDL];
@@ -663,19 +664,21 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
     if (use_acc) {
       __syncthreads();  // the task's contributions are in LDS
       const int A = b.sp_A[P], B = b.sp_B[P];
-      const long long N = b.nao;
+      long long N, msh;
+      mol_layout(b, A, N, msh);
       const long long aoA = b.shell_ao[A], aoB = b.shell_ao[B];
       const int cC = w.acc_colsC, cD = w.acc_colsD;
-      const int* laoC = b.lao_ao + w.lao_offC;
-      const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD : w.lao_offC);
+      const int* mlao = b.mol_lao + b.shell_mol[A] * (ASUKA_LMAX + 1);  // the molecule's AO lists
+      const int* laoC = b.lao_ao + w.lao_offC + mlao[PL::lc];
+      const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD + mlao[PL::ld] : w.lao_offC + mlao[PL::lc]);
       const int cDe = cD > 0 ? cD : cC;
       for (int t2 = threadIdx.x; t2 < nacc * nden; t2 += nthr) {
         const double v = s_acc[t2];
         if (v == 0.0) continue;
         s_acc[t2] = 0.0;
         const bool second = t2 >= nacc;
-        double* Jo = second ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
-        double* Ko = second ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+        double* Jo = shifted(second ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
+        double* Ko = shifted(second ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
         int r = second ? t2 - nacc : t2;
         if (r < NA * NB) {
           atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], v);

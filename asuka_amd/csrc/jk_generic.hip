@@ -93,10 +93,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // the same layout 2 * nblk further on, filled from D2 and flushed to J2, K2
     const int nden = (digest && o.D2) ? 2 : 1;
     if (digest) {
-      const long long N = b.nao;
+      long long N, msh;  // the molecule's D/J/K block
+      mol_layout(b, A, N, msh);
       const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
       for (int di = 0; di < nden; ++di) {
-        const double* Dm = di ? o.D2 : o.D;
+        const double* Dm = (di ? o.D2 : o.D) + msh;
         const int so = di * 2 * nblk;
         for (int e = lane; e < nblk; e += kWave) Lj_ab[so + e] = 0.0;
         for (int e = lane; e < nCD; e += kWave) Dcd[so + e] = Dm[(aoC + e / nD) * N + aoD + e % nD];
@@ -252,14 +253,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     if (digest) {
       // flush the block sums: one global atomic per block entry
       __syncthreads();
-      const long long N = b.nao;
+      long long N, msh;
+      mol_layout(b, A, N, msh);
       const int aoA = b.shell_ao[A], aoB = b.shell_ao[B], aoC = b.shell_ao[C], aoD = b.shell_ao[Dsh];
       const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
       const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
       const double fk = (bk ? 2.0 : 1.0) * (o.mode == MODE_FOCK ? -0.5 : 1.0);
       for (int di = 0; di < nden; ++di) {
-        double* Jo = di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J);
-        double* Ko = di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K);
+        double* Jo = shifted(di ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
+        double* Ko = shifted(di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
         const double* L = Lj_ab + di * 2 * nblk;  // this density's block sums
         const double* Ljab = L;
         const double* Ljcd = Ljab + nAB;

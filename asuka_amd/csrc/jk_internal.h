@@ -54,6 +54,10 @@ struct DevBasis {
   const PrimPairF* ppf;      // same indexing as pp
   const int* shell_lao;      // (nshell) first AO of the shell in the compact numbering of its l
   const int* lao_ao;         // compact (l, index) -> AO, concatenated over l
+  const int* shell_mol;      // (nshell) molecule of the shell (0 unless a batch context)
+  const int* mol_lao;        // (nmol, LMAX+1) start of a molecule's AO list within its l part of lao_ao
+  const int* mol_n;          // (nmol) row stride of the molecule's D/J/K block
+  const long long* mol_shift;  // (nmol) element offset of that block (see mol_layout)
   const double* Q;           // (nsp) Schwarz bounds
   const double* Dsp;         // (nsp) max |D| per shell-pair block
   int nao;
@@ -140,6 +144,21 @@ struct EriOut {
 };
 
 constexpr int kCountStride = 16;
+
+// D/J/K layout: element (a, b) of a matrix X, for AOs a, b (global indices) of
+// the molecule that shell A belongs to, is X[shift + a * stride + b]. One
+// molecule: stride nao, shift 0 (plain row-major nao x nao). A batch context
+// packs each molecule's nao_m x nao_m block at its own base:
+// stride nao_m, shift base_m - off_m (nao_m + 1), off_m its first AO.
+__device__ inline void mol_layout(const DevBasis& b, int A, long long& stride, long long& shift) {
+  const int m = b.shell_mol[A];
+  stride = b.mol_n[m];
+  shift = b.mol_shift[m];
+}
+template <typename T>
+__device__ inline T* shifted(T* p, long long sh) {
+  return p ? p + sh : p;
+}
 
 __host__ __device__ constexpr inline int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
 

@@ -364,12 +364,94 @@ def direct_fock_rhf(
     return F
 
 
+@dataclass
+class DirectJKBatchContext:
+    """Several independent molecules in one context (asuka_jk_batch_create):
+    one launch per class pair covers the whole batch."""
+
+    naos: list
+    device: Any
+    handle: int
+    ntasks: int
+    _keep: tuple = field(default=(), repr=False)
+
+    def __del__(self):
+        h = getattr(self, "handle", 0)
+        if h:
+            try:
+                _lib.lib().asuka_jk_ctx_destroy(ctypes.c_void_p(h))
+            except Exception:
+                pass
+            self.handle = 0
+
+    def set_tuning(self, table: list[dict]) -> None:
+        """Apply a tuning table (e.g. from one molecule's
+        :meth:`DirectJKContext.autotune` in the same basis)."""
+        DirectJKContext.set_tuning(self, table)
+
+    def tuning(self) -> list[dict]:
+        return DirectJKContext.tuning(self)
+
+
+def make_direct_jk_batch_context(ao_bases, *, eps_schwarz: float = 1e-12, eps_density: float | None = None,
+                                 tau_mixed: float = 0.0, streams: int | None = None) -> DirectJKBatchContext:
+    """One context for a batch of independent molecules (one AO basis each)."""
+    torch = _torch()
+    if eps_density is None:
+        env = os.environ.get("ASUKA_DIRECT_JK_EPS_DENSITY", "").strip()
+        eps_density = float(env) if env else max(float(eps_schwarz), 1e-10)
+    if streams is None:
+        env = os.environ.get("ASUKA_DIRECT_JK_STREAMS", "").strip()
+        streams = int(env) if env else 0
+    ao_bases = list(ao_bases)
+    if not ao_bases:
+        raise ValueError("ao_bases must hold at least one basis")
+    descs, keeps = [], []
+    for ao in ao_bases:
+        if int(np.asarray(ao.shell_l).size) == 0:
+            raise ValueError("AO basis has zero shells")
+        d, k = _basis_desc(ao)
+        descs.append(d)
+        keeps.append(k)
+    arr = (ctypes.POINTER(_lib.BasisDesc) * len(descs))(*[ctypes.pointer(d) for d in descs])
+    opts = _lib.JKOpts(float(eps_schwarz), float(eps_density), float(tau_mixed), int(streams))
+    device = torch.device("cuda", torch.cuda.current_device())
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().asuka_jk_batch_create(len(descs), arr, ctypes.byref(opts),
+                                                ctypes.c_void_p(_lib.stream_handle()), ctypes.byref(h)))
+    naos = np.zeros(len(descs), dtype=np.int32)
+    _lib.check(_lib.lib().asuka_jk_batch_info(h, naos.ctypes.data_as(_lib._i32p), len(descs)))
+    info = np.zeros(8, dtype=np.int64)
+    _lib.check(_lib.lib().asuka_jk_ctx_info(h, info.ctypes.data_as(_lib._i64p)))
+    return DirectJKBatchContext(naos=[int(n) for n in naos], device=device, handle=int(h.value), ntasks=int(info[3]),
+                                _keep=(descs, keeps))
+
+
 def direct_JK_batched(ctxs, Ds, *, want_J: bool = True, want_K: bool = True, mixed_precision: bool | None = None,
                       out=None):
-    """J and K of several independent molecules in one call (one context per
-    molecule, issued back to back on the current stream with no host sync, so
-    a HIP graph can capture the whole batch). Returns a list of (J, K)."""
+    """J and K of several independent molecules in one call, as a list of
+    (J, K). ``ctxs`` is either a :class:`DirectJKBatchContext` (one launch per
+    class pair for the whole batch, asuka_jk_batch_build) or a list of
+    per-molecule contexts (their builds issued in turn, asuka_jk_build_batched).
+    No host synchronisation either way, so a HIP graph can capture the batch."""
     torch = _torch()
+    if isinstance(ctxs, DirectJKBatchContext):
+        bc = ctxs
+        n = len(bc.naos)
+        if len(Ds) != n:
+            raise ValueError("Ds must have one density per molecule")
+        Dd = [_as_device_matrix(torch, D, m, "D", bc.device) for m, D in zip(bc.naos, Ds)]
+        if out is None:
+            out = [(torch.empty((m, m), dtype=torch.float64, device=bc.device) if want_J else None,
+                    torch.empty((m, m), dtype=torch.float64, device=bc.device) if want_K else None) for m in bc.naos]
+        arr = ctypes.c_void_p * n
+        dp = arr(*[ctypes.c_void_p(_lib.ptr(d)) for d in Dd])
+        jp = arr(*[ctypes.c_void_p(_lib.ptr(o[0])) for o in out])
+        kp = arr(*[ctypes.c_void_p(_lib.ptr(o[1])) for o in out])
+        flags = (_lib.WANT_J if want_J else 0) | (_lib.WANT_K if want_K else 0) | _flags(mixed_precision)
+        _lib.check(_lib.lib().asuka_jk_batch_build(ctypes.c_void_p(bc.handle), dp, jp, kp, flags,
+                                                   ctypes.c_void_p(_lib.stream_handle(bc.device))))
+        return out
     n = len(ctxs)
     if len(Ds) != n:
         raise ValueError("Ds must have one density per context")
@@ -414,5 +496,5 @@ def eri_tiles(ctx: DirectJKContext, spab, spcd):
     return [host[o:o + s].reshape(shp) for o, s, shp in zip(off, sizes, shapes)]
 
 
-__all__ = ["DirectJKContext", "direct_JK", "direct_JK_batched", "direct_JK_multi", "direct_fock_rhf", "eri_tiles",
-           "make_direct_jk_context"]
+__all__ = ["DirectJKBatchContext", "DirectJKContext", "direct_JK", "direct_JK_batched", "direct_JK_multi",
+           "direct_fock_rhf", "eri_tiles", "make_direct_jk_batch_context", "make_direct_jk_context"]

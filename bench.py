@@ -52,75 +52,80 @@ CONFIGS = {
 def run_batched(args, cfg) -> int:
     """Batched workload: 32 independent clusters (sizes default_rng(3).integers(1,
     17, 32), placement seeds 1000 + i), random orthonormal C with 5 orbitals per
-    water (seed 2000 + i), one batched J/K build per step, eager and replayed
-    from a captured HIP graph, with the default launch shapes and then with
-    the shapes the autotuner picks on the largest molecule (applied to all)."""
+    water (seed 2000 + i), one batched J/K build per step on a batch context
+    (one launch per class pair for the whole batch), eager and replayed from a
+    captured HIP graph, with the default launch shapes and with the shapes the
+    autotuner picks on the largest molecule. The per-molecule-context path
+    (builds issued in turn) is timed beside it."""
     import torch
 
     from asuka_amd.basis import build_ao_basis_cart
-    from asuka_amd.hf.direct_jk import direct_JK_batched, make_direct_jk_context
+    from asuka_amd.hf.direct_jk import direct_JK_batched, make_direct_jk_batch_context, make_direct_jk_context
     from asuka_amd.molecule import water_cluster
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     sizes = np.random.default_rng(cfg["seed"]).integers(1, 17, cfg["nmol"])
-    ctxs, Ds = [], []
-    t0 = time.perf_counter()
+    aos, Ds = [], []
     for i, n in enumerate(sizes):
-        mol = water_cluster(int(n), 1000 + i, cfg["basis"])
-        ao, _ = build_ao_basis_cart(mol)
-        ctxs.append(make_direct_jk_context(ao, eps_schwarz=cfg["eps"]))
+        ao, _ = build_ao_basis_cart(water_cluster(int(n), 1000 + i, cfg["basis"]))
+        aos.append(ao)
         C = np.random.default_rng(2000 + i).standard_normal((ao.nao, 5 * int(n)))
         C, _ = np.linalg.qr(C)
         Ds.append(torch.tensor(2.0 * C @ C.T, device=dev))
+    t0 = time.perf_counter()
+    bc = make_direct_jk_batch_context(aos, eps_schwarz=cfg["eps"])
     t_ctx = time.perf_counter() - t0
-    nq = sum(c.ntasks for c in ctxs)
-    log(f"batch: {len(ctxs)} molecules, {sum(c.nao for c in ctxs)} AOs, {nq} Schwarz-screened quartets "
-        f"(contexts {t_ctx:.1f}s)")
-    out = direct_JK_batched(ctxs, Ds)
+    t0 = time.perf_counter()
+    ctxs = [make_direct_jk_context(ao, eps_schwarz=cfg["eps"]) for ao in aos]
+    t_ctxs = time.perf_counter() - t0
+    nq = bc.ntasks
+    log(f"batch: {len(aos)} molecules, {sum(bc.naos)} AOs, {nq} Schwarz-screened quartets "
+        f"(batch context {t_ctx:.1f}s, per-molecule contexts {t_ctxs:.1f}s)")
 
-    def measure():
+    def measure(target):
+        out = direct_JK_batched(target, Ds)
         for _ in range(max(0, args.warmup)):
-            direct_JK_batched(ctxs, Ds, out=out)
+            direct_JK_batched(target, Ds, out=out)
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            direct_JK_batched(ctxs, Ds, out=out)
+            direct_JK_batched(target, Ds, out=out)
         torch.cuda.synchronize(dev)
         s_eager = (time.perf_counter() - t0) / args.steps
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            direct_JK_batched(ctxs, Ds, out=out)
+            direct_JK_batched(target, Ds, out=out)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize(dev)
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
-            direct_JK_batched(ctxs, Ds, out=out)
+            direct_JK_batched(target, Ds, out=out)
         graph.replay()
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             graph.replay()
         torch.cuda.synchronize(dev)
-        return s_eager, (time.perf_counter() - t0) / args.steps
+        return s_eager, (time.perf_counter() - t0) / args.steps, [x.clone() for p in out for x in p if x is not None]
 
-    s_eager0, s_graph0 = measure()
-    ref = [x.clone() for pair in out for x in pair if x is not None]
-    log(f"batch default shapes: eager {s_eager0 * 1e3:.2f} ms, graph {s_graph0 * 1e3:.2f} ms")
-    # autotune on the largest molecule, apply its table to the whole batch
-    big = int(np.argmax([c.nao for c in ctxs]))
+    s_eager_m, s_graph_m, ref = measure(ctxs)
+    log(f"per-molecule contexts: eager {s_eager_m * 1e3:.2f} ms, graph {s_graph_m * 1e3:.2f} ms")
+    s_eager0, s_graph0, got0 = measure(bc)
+    d0 = max(float((a - b).abs().max()) for a, b in zip(got0, ref))
+    log(f"batch context, default shapes: eager {s_eager0 * 1e3:.2f} ms, graph {s_graph0 * 1e3:.2f} ms "
+        f"(max |dJ|,|dK| vs per-molecule contexts {d0:.1e})")
+    # autotune on the largest molecule, apply its table to the batch
+    big = int(np.argmax([a.nao for a in aos]))
     t0 = time.perf_counter()
     table = ctxs[big].autotune(Ds[big])
-    for i, c in enumerate(ctxs):
-        if i != big:
-            c.set_tuning(table)
+    bc.set_tuning(table)
     t_tune = time.perf_counter() - t0
-    s_eager, s_graph = measure()
-    got = [x for pair in out for x in pair if x is not None]
+    s_eager, s_graph, got = measure(bc)
     dmax = max(float((a - b).abs().max()) for a, b in zip(got, ref))
-    log(f"batch autotuned shapes: eager {s_eager * 1e3:.2f} ms, graph {s_graph * 1e3:.2f} ms "
-        f"(tuning {t_tune:.2f}s, max |dJ|,|dK| vs default shapes {dmax:.1e})")
+    log(f"batch context, autotuned shapes: eager {s_eager * 1e3:.2f} ms, graph {s_graph * 1e3:.2f} ms "
+        f"(tuning {t_tune:.2f}s, max |dJ|,|dK| vs per-molecule contexts {dmax:.1e})")
     res = {
         "metric": "batched J/K build: Schwarz-screened shell quartets/s over the batch (graph replay)",
         "value": nq / s_graph,
@@ -135,15 +140,18 @@ def run_batched(args, cfg) -> int:
         "dtype": "f64",
         "data": "synthetic (seeded water clusters, random orthonormal occupied orbitals)",
         "config": {"workload": cfg["workload"], "sizes": [int(n) for n in sizes], "basis": cfg["basis"],
-                   "eps_schwarz": cfg["eps"]},
+                   "eps_schwarz": cfg["eps"], "launches": "one per class pair for the whole batch",
+                   "launch_shapes": "autotuned"},
         "s_per_batch_eager": s_eager,
         "s_per_batch_graph": s_graph,
         "graph_speedup": s_eager / s_graph,
         "default_shapes": {"s_per_batch_eager": s_eager0, "s_per_batch_graph": s_graph0},
         "autotuned_vs_default": s_graph0 / s_graph,
         "autotune_s": t_tune,
-        "autotune_table_from": f"molecule {big} ({ctxs[big].nao} AOs), applied to all",
-        "max_abs_dJK_tuned_vs_default": dmax,
+        "autotune_table_from": f"molecule {big} ({aos[big].nao} AOs), applied to the batch",
+        "per_molecule_contexts": {"s_per_batch_eager": s_eager_m, "s_per_batch_graph": s_graph_m},
+        "batch_context_vs_per_molecule": s_graph_m / s_graph,
+        "max_abs_dJK_vs_per_molecule_contexts": max(d0, dmax),
         "context_build_s": t_ctx,
     }
     print(json.dumps(res), flush=True)

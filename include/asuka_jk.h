@@ -182,6 +182,25 @@ int asuka_int1e(asuka_jk_ctx* ctx, double* S, double* T, double* V, void* stream
  * batch can be captured in one HIP graph.
  * The replacement of the per-molecule loop over direct_JK calls (the batched
  * multi-molecule workload); the reference has no batched entry point. */
+/* A batch context: nmol independent molecules (one basis each, in the layout
+ * of asuka_jk_ctx_create) in one context. Shell pairs and quartets stay
+ * within a molecule, every class pair of the whole batch is one launch fed by
+ * its device work queue, and each molecule's D/J/K is a packed block of a
+ * context-owned buffer. Use it with asuka_jk_batch_build (the single-molecule
+ * entry points reject it). Counterpart of the per-molecule loop over
+ * direct_JK (the reference has no batched entry point). */
+int asuka_jk_batch_create(int nmol, const asuka_basis_desc* const* bases, const asuka_jk_opts* opts, void* stream,
+                          asuka_jk_ctx** out);
+/* AO count of each molecule of a batch context (min(cap, nmol) written); returns nmol. */
+int asuka_jk_batch_info(const asuka_jk_ctx* ctx, int32_t* mol_nao, int cap);
+/* J/K of every molecule of a batch context: D[m], J[m], K[m] are the caller's
+ * row-major nao_m x nao_m device matrices (J or K may be NULL when not wanted).
+ * The densities are copied into the context's packed blocks, one build covers
+ * the batch, and the symmetrised blocks are copied out. Stream-ordered, no host
+ * synchronisation or allocation (graph-capturable). */
+int asuka_jk_batch_build(asuka_jk_ctx* ctx, const double* const* D, double* const* J, double* const* K, int flags,
+                         void* stream);
+
 int asuka_jk_build_batched(int nmol, asuka_jk_ctx* const* ctxs, const double* const* D, double* const* J,
                            double* const* K, int flags, void* stream);
 
