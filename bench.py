@@ -527,7 +527,7 @@ def main() -> int:
         if not args.no_cpu:
             threads = len(os.sched_getaffinity(0))
             # calibrate on a small sample, then size the sample to ~cpu_seconds
-            cab, ccd = sample_screened(Q, Dsp, cfg["eps"], ctx.eps_density, 2000, rng)
+            cab, ccd = sample_screened(Q, Dsp, cfg["eps"], ctx.eps_density, 20000, rng)
             t1 = time.perf_counter()
             O.direct_jk_tasks(octx, D_h, cab, ccd, hp=False, threads=threads)
             rate0 = cab.size / (time.perf_counter() - t1)
