@@ -61,6 +61,21 @@ class JKOpts(ctypes.Structure):
     ]
 
 
+class JKStats(ctypes.Structure):
+    _fields_ = [
+        ("class_pairs", ctypes.c_int64),
+        ("schwarz_quartets", ctypes.c_int64),
+        ("quartets", ctypes.c_int64),
+        ("quartets_fp32", ctypes.c_int64),
+        ("prim_quartets", ctypes.c_int64),
+        ("prim_quartets_fp32", ctypes.c_int64),
+        ("flops_efm", ctypes.c_double),
+        ("flops_efm_fp32", ctypes.c_double),
+        ("last_ms", ctypes.c_double),
+        ("reserved", ctypes.c_int64 * 7),
+    ]
+
+
 # (name, restype, argtypes) for every symbol of include/asuka_jk.h
 SYMBOLS = [
     ("asuka_jk_version", ctypes.c_int, []),
@@ -79,6 +94,7 @@ SYMBOLS = [
     ("asuka_jk_class_stats", ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
     ("asuka_jk_last_timing", ctypes.c_int, [_vp, _f64p, _i64p]),
     ("asuka_jk_last_counts", ctypes.c_int, [_vp, _i64p, ctypes.c_int, _vp]),
+    ("asuka_jk_stats", ctypes.c_int, [_vp, ctypes.POINTER(JKStats), _vp]),
     ("asuka_jk_eri_tiles", ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_int1e", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("asuka_jk_build_batched", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),

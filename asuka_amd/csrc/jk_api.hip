@@ -1396,6 +1396,46 @@ int asuka_jk_last_counts(asuka_jk_ctx* ctx, int64_t* rows, int cap, void* stream
   return n;
 }
 
+int asuka_jk_stats(asuka_jk_ctx* ctx, asuka_jk_stats_out* out, void* stream) {
+  if (!ctx || !out) return fail(ASUKA_E_INVALID, "ctx and out must be non-null");
+  *out = asuka_jk_stats_out{};
+  const int n = (int)ctx->cps.size();
+  out->class_pairs = n;
+  std::vector<unsigned long long> h(kCountStride * (size_t)std::max(1, n), 0ull);
+  if (n > 0) {
+    HIP_CHECK(hipMemcpyAsync(h.data(), ctx->d_counters, sizeof(unsigned long long) * kCountStride * n,
+                             hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  }
+  for (int i = 0; i < n; ++i) {
+    const ClassPairHost& cp = ctx->cps[i];
+    const unsigned long long* c = h.data() + (size_t)kCountStride * i;
+    out->schwarz_quartets += cp.nquartets;
+    // EFM-v1: per primitive quartet 24 + n (24 + 15 (nmax+1)(mmax+1) + 6 N1 + 4 ncomp);
+    // per quartet 2 ncomp (jt + kt) for the digestion
+    const int nr = (cp.la + cp.lb + cp.lc + cp.ld) / 2 + 1;
+    const double ncomp = (double)ncart_c(cp.la) * ncart_c(cp.lb) * ncart_c(cp.lc) * ncart_c(cp.ld);
+    const double n1 = (double)(cp.la + 1) * (cp.lb + 1) * (cp.lc + 1) * (cp.ld + 1);
+    const double per_prim = 24.0 + nr * (24.0 + 15.0 * (cp.la + cp.lb + 1) * (cp.lc + cp.ld + 1) + 6.0 * n1 + 4.0 * ncomp);
+    for (int bin = 0; bin < 8; ++bin) {
+      const int ab = (bin >> 2) & 1, cd = (bin >> 1) & 1, pq = bin & 1;
+      const int jt = 1 + pq, kt = 1 + cd + ab + (ab & cd);
+      out->quartets += (int64_t)c[bin];
+      out->flops_efm += 2.0 * ncomp * (jt + kt) * (double)c[bin];
+    }
+    out->prim_quartets += (int64_t)c[8];
+    out->quartets_fp32 += (int64_t)c[9];
+    out->prim_quartets_fp32 += (int64_t)c[10];
+    out->flops_efm += per_prim * (double)c[8];
+    out->flops_efm_fp32 += per_prim * (double)c[10];
+  }
+  if (ctx->ev_start && ctx->ev_stop && hipEventQuery(ctx->ev_stop) != hipErrorNotReady) {
+    float f = 0.0f;
+    if (hipEventElapsedTime(&f, ctx->ev_start, ctx->ev_stop) == hipSuccess) out->last_ms = (double)f;
+  }
+  return ASUKA_OK;
+}
+
 int asuka_jk_eri_tiles(asuka_jk_ctx* ctx, int64_t nq, const int32_t* spab, const int32_t* spcd, const int64_t* out_off,
                        double* out, int flags, void* stream) {
   if (!ctx || nq < 0 || (nq > 0 && (!spab || !spcd || !out_off || !out)))

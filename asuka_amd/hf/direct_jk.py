@@ -91,6 +91,15 @@ class DirectJKContext:
                         "prim_quartets": int(r[5]), "prim_roots": int(r[6])})
         return out
 
+    def stats(self) -> dict:
+        """Work summary (asuka_jk_stats): Schwarz-screened quartets and, from the
+        last build with the count flag, the quartets and primitive quartets
+        evaluated, their FP32 share and the EFM-v1 FLOPs."""
+        out = _lib.JKStats()
+        _lib.check(_lib.lib().asuka_jk_stats(ctypes.c_void_p(self.handle), ctypes.byref(out),
+                                             ctypes.c_void_p(_lib.stream_handle(self.device))))
+        return {k: getattr(out, k) for k, _ in _lib.JKStats._fields_ if k != "reserved"}
+
     def autotune(self, D, *, mixed_precision: bool | None = None, reps: int = 2) -> list[dict]:
         """Time every class pair's launch on density D and keep the fastest
         launch shape per class pair (see :meth:`tuning`); the class pairs are

@@ -27,7 +27,7 @@ import ctypes
 import numpy as np
 
 from .. import _lib
-from .direct_jk import DirectJKContext, _as_device_matrix, _torch, direct_JK
+from .direct_jk import DirectJKContext, _as_device_matrix, _torch, direct_JK, direct_JK_multi
 
 
 def _sym(torch, X):
@@ -123,9 +123,10 @@ class DirectJKFunction:
 
             class _F(torch.autograd.Function):
                 @staticmethod
-                def forward(fctx, D, R, jk_ctx):
-                    J, K = direct_JK(jk_ctx, D)
+                def forward(fctx, D, R, jk_ctx, mixed_precision):
+                    J, K = direct_JK(jk_ctx, D, mixed_precision=mixed_precision)
                     fctx.jk_ctx = jk_ctx
+                    fctx.mixed = mixed_precision
                     fctx.save_for_backward(D.detach())
                     return J, K
 
@@ -141,22 +142,23 @@ class DirectJKFunction:
                     if fctx.needs_input_grad[0]:
                         # J and K are linear in D with symmetric kernels: the
                         # adjoint of D -> J(D) is J itself, of D -> K(D) is K.
-                        dJ, _ = direct_JK(jk_ctx, gJ, want_K=False)
-                        _, dK = direct_JK(jk_ctx, gK, want_J=False)
+                        # Both come from one ERI pass of the two-density build.
+                        dJ, _, _, dK = direct_JK_multi(jk_ctx, gJ, gK, mixed_precision=fctx.mixed)
                         dD = dJ + dK
                     if fctx.needs_input_grad[1]:
                         dR = direct_jk_grad(jk_ctx, D, WJ=gJ, WK=gK)
-                    return dD, dR, None
+                    return dD, dR, None, None
 
             cls._fn = _F
         return cls._fn
 
 
-def direct_jk_autograd(jk_ctx: DirectJKContext, D, R):
+def direct_jk_autograd(jk_ctx: DirectJKContext, D, R, *, mixed_precision: bool | None = None):
     """(J, K) with autograd through D and the nuclear coordinates R (natm, 3),
     which must be the coordinates the context was built at (the context holds
     the basis at that geometry; the backward evaluates the analytic derivative
-    there)."""
+    there). ``mixed_precision`` applies to the J/K builds of the forward and of
+    dL/dD; the derivative contraction for dL/dR is FP64."""
     torch = _torch()
     _need_atoms(jk_ctx)
     R0 = getattr(jk_ctx, "atom_coords_bohr", None)
@@ -164,7 +166,7 @@ def direct_jk_autograd(jk_ctx: DirectJKContext, D, R):
         raise ValueError("R must be the coordinates the context was built at")
     if not isinstance(D, torch.Tensor):
         D = torch.as_tensor(np.asarray(D, dtype=np.float64), device=jk_ctx.device)
-    return DirectJKFunction.get().apply(D, R, jk_ctx)
+    return DirectJKFunction.get().apply(D, R, jk_ctx, mixed_precision)
 
 
 __all__ = ["DirectJKFunction", "direct_jk_autograd", "direct_jk_grad", "energy_weighted_density", "int1e_grad",

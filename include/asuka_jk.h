@@ -149,6 +149,31 @@ int asuka_jk_shell_pairs(const asuka_jk_ctx* ctx, int32_t* sp_A, int32_t* sp_B, 
  * of rows (>= 0) or an error code. */
 int asuka_jk_class_stats(const asuka_jk_ctx* ctx, int64_t* rows, int cap);
 
+/* Work summary of a context (asuka_jk_stats): the Schwarz-screened quartet
+ * count, and from the last build run with ASUKA_JK_COUNT the quartets
+ * evaluated (after density screening), their FP32 share (ASUKA_JK_MIXED),
+ * their primitive quartets (counted before primitive-pair screening) and
+ * the algorithmic FLOPs by the EFM-v1 convention (ERI + digestion; the FP32
+ * part of the ERI FLOPs separately); last_ms is the duration of the ERI
+ * launches of the last build run with ASUKA_JK_TIME (0 if none finished). */
+typedef struct {
+  int64_t class_pairs;
+  int64_t schwarz_quartets;
+  int64_t quartets;
+  int64_t quartets_fp32;
+  int64_t prim_quartets;
+  int64_t prim_quartets_fp32;
+  double flops_efm;
+  double flops_efm_fp32;
+  double last_ms;
+  int64_t reserved[7];
+} asuka_jk_stats_out;
+
+/* Replaces the stats/profile dicts of direct_JK (asuka/hf/direct_jk.py:1687-1700,
+ * :1958-1989) and the cost model of asuka/hf/direct_jk_cost.py:134-294.
+ * Synchronises on stream to read the counters. */
+int asuka_jk_stats(asuka_jk_ctx* ctx, asuka_jk_stats_out* out, void* stream);
+
 /* Counters of the last build run with ASUKA_JK_COUNT, one row of 16 int64 per
  * class pair in asuka_jk_class_stats order: [0..7] quartets evaluated after
  * density screening, binned by (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair)

@@ -29,6 +29,14 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) == bound
 
 
+def test_struct_layouts():
+    import ctypes
+
+    # asuka_jk_opts: 3 doubles + int32 streams + int32[7]; asuka_jk_stats_out: 6 int64 + 3 doubles + int64[7]
+    assert ctypes.sizeof(_lib.JKOpts) == 56
+    assert ctypes.sizeof(_lib.JKStats) == 128
+
+
 def test_version_and_error_paths():
     L = _lib.lib()
     assert L.asuka_jk_version() == 1

@@ -265,3 +265,37 @@ def test_mixed_precision_scf_energy(gpu):
     emx = run_hf_df(mol, two_e_backend="direct", eps_schwarz=1e-12, mixed_precision=True)
     assert e64.scf.converged and emx.scf.converged
     assert abs(emx.scf.e_tot - e64.scf.e_tot) < 1e-6
+
+
+def test_stats_match_efm_model(gpu):
+    """asuka_jk_stats after a counted build equals the EFM-v1 model evaluated
+    in Python on the per-class counters (asuka_amd/metrics.py)."""
+    import ctypes
+
+    from asuka_amd import _lib, metrics
+    from asuka_amd.hf.direct_jk import direct_JK
+
+    torch = gpu
+    mol = water_cluster(2, 3, "cc-pVTZ")
+    ao, ctx = _ctx(mol)
+    D = torch.tensor(_density(ao.nao, 10, 5), device="cuda")
+    direct_JK(ctx, D)  # warm
+    J = torch.empty_like(D)
+    K = torch.empty_like(D)
+    L = _lib.lib()
+    sptr = ctypes.c_void_p(_lib.stream_handle())
+    _lib.check(L.asuka_jk_build(ctypes.c_void_p(ctx.handle), _lib.ptr(D), _lib.ptr(J), _lib.ptr(K),
+                                _lib.WANT_J | _lib.WANT_K | _lib.MIXED | _lib.COUNT | _lib.TIME, sptr))
+    st = ctx.stats()
+    ncp = len(ctx.class_stats())
+    rows = np.zeros((ncp, 16), dtype=np.int64)
+    L.asuka_jk_last_counts(ctypes.c_void_p(ctx.handle), rows.ctypes.data_as(_lib._i64p), ncp, sptr)
+    cls = [s["class"] for s in ctx.class_stats()]
+    work = metrics.build_work([(c, rows[i, :8].tolist(), int(rows[i, 8])) for i, c in enumerate(cls)])
+    assert st["class_pairs"] == ncp and st["schwarz_quartets"] == ctx.ntasks
+    assert st["quartets"] == work.quartets and st["prim_quartets"] == work.prim_quartets
+    assert abs(st["flops_efm"] - work.flops) <= 1e-9 * work.flops
+    f32 = sum(metrics.flop_eri(*c, int(rows[i, 10])) for i, c in enumerate(cls))
+    assert abs(st["flops_efm_fp32"] - f32) <= 1e-9 * max(1.0, f32)
+    assert 0 < st["quartets_fp32"] < st["quartets"] <= st["schwarz_quartets"]
+    assert st["last_ms"] > 0.0
