@@ -47,6 +47,9 @@ static constexpr size_t kRowAccSmallBytes = 40 * 1024;
 static constexpr double kSqrt2Pi52 = 5.91496717279561287782;
 // primitive pairs with Q_p * max Q_p below this are dropped (with eps_schwarz > 0)
 static constexpr double kPrimPairEps = 1e-20;
+// primitive quartets with Q_p Q_q below this are skipped by the register
+// kernels (with eps_schwarz > 0)
+static constexpr double kPrimQuartetEps = 1e-17;
 
 static int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -424,6 +427,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
           f.PAx = r.Px - a3[0];
           f.PAy = r.Py - a3[1];
           f.PAz = r.Pz - a3[2];
+          f.q = 1e300;  // no bound yet (set by primitive-pair screening)
           ctx->ppf.push_back(f);
         }
       }
@@ -591,18 +595,25 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     std::vector<PrimPair> pp2;
     std::vector<PrimPairF> ppf2;
     std::vector<int> start2(ctx->nsp), count2(ctx->nsp);
+    std::vector<int> keep;
     for (int sp = 0; sp < ctx->nsp; ++sp) {
       start2[sp] = (int)pp2.size();
+      keep.clear();
       for (int t = ctx->pp_start[sp]; t < ctx->pp_start[sp] + ctx->pp_count[sp]; ++t)
-        if (Qp[t] * qmax >= kPrimPairEps) {
-          pp2.push_back(ctx->pp[t]);
-          ppf2.push_back(ctx->ppf[t]);
-        }
+        if (Qp[t] * qmax >= kPrimPairEps) keep.push_back(t);
+      // largest bound first: the register kernels stop a primitive loop at the
+      // first pair whose bound is too small (primitive-quartet screening)
+      std::stable_sort(keep.begin(), keep.end(), [&](int x, int y) { return Qp[x] > Qp[y]; });
+      for (int t : keep) {
+        pp2.push_back(ctx->pp[t]);
+        ppf2.push_back(ctx->ppf[t]);
+        ppf2.back().q = Qp[t];
+      }
       count2[sp] = (int)pp2.size() - start2[sp];
     }
     ctx->nprim_dropped = npp - (long long)pp2.size();
     ctx->pp_count0 = ctx->pp_count;
-    if (ctx->nprim_dropped > 0) {
+    {  // the screened, bound-sorted tables replace the first ones
       ctx->pp.swap(pp2);
       ctx->ppf.swap(ppf2);
       ctx->pp_start.swap(start2);
@@ -765,8 +776,22 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
               // the row's wave units (cut into row tasks below)
               const int nu = (int)((len + 63) / 64);
               cp.rows.emplace_back((int)(units.size() - cp.unit_start), nu);
-              for (long long c0 = 0; c0 < len; c0 += 64)
-                units.push_back(WaveUnit{P, kg.col_start + (int)c0, (int)std::min<long long>(64, len - c0), 0});
+              for (long long c0 = 0; c0 < len; c0 += 64) {
+                const int nc = (int)std::min<long long>(64, len - c0);
+                int nabu = ctx->pp_count[P];
+                if (ctx->eps_schwarz > 0.0) {  // primitive-quartet cut of the bra loop (pairs sorted by bound)
+                  double qw = 0.0;
+                  for (int t = 0; t < nc; ++t) {
+                    const int kq = kg.pairs[c0 + t];
+                    if (ctx->pp_count[kq] > 0) qw = std::max(qw, ctx->ppf[ctx->pp_start[kq]].q);
+                  }
+                  const double thr = qw > 0.0 ? kPrimQuartetEps / qw : 1e308;
+                  int k = 0;
+                  while (k < nabu && ctx->ppf[ctx->pp_start[P] + k].q >= thr) ++k;
+                  nabu = k;
+                }
+                units.push_back(WaveUnit{P, kg.col_start + (int)c0, nc, nabu});
+              }
             }
           }
           if (off == 0) continue;
@@ -1075,6 +1100,7 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
     o.K2 = outs[3];
   }
   o.eps_density = ctx->eps_density;
+  o.eps_prim = ctx->eps_schwarz > 0.0 ? kPrimQuartetEps : 0.0;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
   o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
@@ -1171,6 +1197,7 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   o.D = D;
   o.F = F;
   o.eps_density = ctx->eps_density;
+  o.eps_prim = ctx->eps_schwarz > 0.0 ? kPrimQuartetEps : 0.0;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
   o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
@@ -1276,6 +1303,7 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
   o.J = scratch;
   o.K = scratch + nn;
   o.eps_density = ctx->eps_density;
+  o.eps_prim = ctx->eps_schwarz > 0.0 ? kPrimQuartetEps : 0.0;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
   o.tau_mixed = ctx->tau_mixed;
   // median of reps timed launches (after one untimed) of class pair i as it is now set up

@@ -171,6 +171,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+
 // Arithmetic helpers for the compute type CT of the recursion (double, or
 // float on the FP32 path).
 __device__ __forceinline__ double fmac(double a, double b, double c) { return fma(a, b, c); }
@@ -210,7 +211,7 @@ struct RowAcc {
 // accumulates in FP64. Geometric differences are formed in FP64 and rounded.
 // `tab` is the NR-root block of the root table in CT (LDS or global).
 template <class PL, int CH, typename CT>
-__device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, bool active,
+__device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
                                           const CT* __restrict__ tab, const RowAcc& acc) {
   using CK = Chunk<PL, CH>;
   constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
@@ -230,7 +231,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   const int A = uni(b.sp_A[P]), B = uni(b.sp_B[P]), C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
   const double Ax = b.shell_xyz[3 * A], Ay = b.shell_xyz[3 * A + 1], Az = b.shell_xyz[3 * A + 2];
   const double Cx = b.shell_xyz[3 * C], Cy = b.shell_xyz[3 * C + 1], Cz = b.shell_xyz[3 * C + 2];
-  const int ab0 = uni(b.pp_start[P]), nab = uni(b.pp_count[P]);
+  const int ab0 = uni(b.pp_start[P]);
   const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
   const PrimPairF* __restrict__ ppf = b.ppf;
 
@@ -238,10 +239,18 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
   for (int i = 0; i < NEL * NFL; ++i) ef[i] = CT(0);
 
+  // Primitive-quartet screening: the primitive pairs of a shell pair are
+  // sorted by their Schwarz bound q (largest first). nab (uniform, from the
+  // wave unit) already leaves out the bra pairs whose q times the largest
+  // ket-pair bound of the unit is below eps_prim; a lane leaves its ket loop
+  // at the first ket pair whose q times the largest bra bound is below it.
+  // Every skipped primitive quartet adds less than eps_prim to each integral.
+  const double kcut = (nab > 0 && o.eps_prim > 0.0) ? o.eps_prim / ppf[ab0].q : 0.0;
   PrimPairF Yn;
   if (ncd > 0) Yn = ppf[cd0];
   for (int j = 0; j < ncd; ++j) {
     const PrimPairF Y = Yn;
+    if (Y.q < kcut) break;  // per lane: the rest of its ket pairs are smaller
     if (j + 1 < ncd) Yn = ppf[cd0 + j + 1];  // next ket pair, in flight during this one
     const CT q = CT(Y.eta), iq = CT(Y.inv_eta), ckq = CT(Y.ck);
     const CT QCx = CT(Y.PAx), QCy = CT(Y.PAy), QCz = CT(Y.PAz);
@@ -516,14 +525,14 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 }
 
 template <class PL, int CH, typename CT>
-__device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp,
+__device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp, int nab,
                                                bool active, const CT* __restrict__ tab, const RowAcc& acc) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH, CT>(b, o, P, Qp, active, tab, acc);
+      run_chunk<PL, CH, CT>(b, o, P, Qp, nab, active, tab, acc);
       return;
     }
-    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, active, tab, acc);
+    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, nab, active, tab, acc);
   }
 }
 
@@ -656,10 +665,12 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       }
       // inactive lanes (tail, screened) run with an empty primitive loop and no
       // atomics so the wave stays converged for the cross-lane reductions
+      // bra primitive pairs this unit evaluates (see run_chunk)
+      const int nab = uni(o.eps_prim > 0.0 ? un.nab : b.pp_count[P]);
       if (lowp)
-        dispatch_chunk<PL, 0, float>(chunk, b, o, P, Qp, active, tab32, acc);
+        dispatch_chunk<PL, 0, float>(chunk, b, o, P, Qp, nab, active, tab32, acc);
       else
-        dispatch_chunk<PL, 0, double>(chunk, b, o, P, Qp, active, tab64, acc);
+        dispatch_chunk<PL, 0, double>(chunk, b, o, P, Qp, nab, active, tab64, acc);
     }
     if (use_acc) {
       __syncthreads();  // the task's contributions are in LDS

@@ -30,7 +30,7 @@ struct __align__(16) PrimPairF {
   double eta, inv_eta, ck;
   double Px, Py, Pz;
   double PAx, PAy, PAz;  // P minus the centre of the pair's first shell
-  double pad;
+  double q;              // Schwarz bound of the primitive pair alone (pairs sorted by it, descending)
 };
 
 enum EriMode : int {
@@ -88,7 +88,8 @@ struct WaveUnit {
   int P;     // bra shell pair
   int col;   // first ket, index into col_pair
   int ncol;  // 1..64
-  int pad;
+  int nab;   // bra primitive pairs to evaluate: those whose bound times the
+             // largest ket-pair bound of the unit reaches the primitive-quartet cut
 };
 
 // A workgroup's share: consecutive wave units [u0, u0 + nu) of one bra row.
@@ -134,6 +135,7 @@ struct EriOut {
   // by symmetry bin (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair), [8] their
   // primitive quartets, [9] quartets evaluated in FP32, [10] their primitive quartets
   unsigned long long* counters;
+  double eps_prim;    // primitive quartets with Q_p Q_q below this are skipped (0: none)
   int mixed;          // allow the FP32 path
   double tau_mixed;   // FP32 when Q_ab Q_cd max|D| < tau_mixed for the whole wave
   // MODE_JK with two densities (UHF/ROHF): each quartet is evaluated once and
