@@ -46,7 +46,7 @@ static constexpr size_t kRowAccSmallBytes = 40 * 1024;
 // sqrt(2 pi^(5/2)), split evenly over the bra and ket primitive-pair factors
 static constexpr double kSqrt2Pi52 = 5.91496717279561287782;
 // primitive pairs with Q_p * max Q_p below this are dropped (with eps_schwarz > 0)
-static constexpr double kPrimPairEps = 1e-20;
+static constexpr double kPrimPairEps = 1e-17;
 // primitive quartets with Q_p Q_q below this are skipped by the register
 // kernels (with eps_schwarz > 0)
 static constexpr double kPrimQuartetEps = 1e-17;
