@@ -39,7 +39,8 @@ static constexpr size_t kRowAccMaxBytes = 72 * 1024;
 // streams a build's class-pair launches are spread over (the caller's included)
 static constexpr int kDefaultStreams = 4;
 static constexpr int kMaxStreams = 8;
-static constexpr size_t kRootTablesMaxBytes = 27 * 1024;
+// LDS the register kernels of a class with nr roots hold for root tables
+static inline size_t root_table_lds(int nr) { return nr <= kLdsRootsMax ? rys_c::lds_table_bytes(nr) : 0; }
 // Accumulators up to this size run with 4-wave workgroups (16-unit tasks),
 // larger ones with 8-wave workgroups (32-unit tasks).
 static constexpr size_t kRowAccSmallBytes = 40 * 1024;
@@ -301,7 +302,8 @@ static bool row_acc_fits(const asuka_jk_ctx* ctx, int la, int lb, int lc, int ld
   *cD = (lc == ld) ? 0 : ctx->nlao[ld];
   const size_t bytes =
       sizeof(double) * (size_t)(ncart_c(la) * ncart_c(lb) + (ncart_c(la) + ncart_c(lb)) * (*cC + *cD));
-  return bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - kRootTablesMaxBytes);
+  const int nr = (la + lb + lc + ld) / 2 + 1;
+  return bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - root_table_lds(nr) - 1024);
 }
 
 extern "C" {
@@ -740,7 +742,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
         const int cC = ctx->nlao[cp.lc], cD = (cp.lc == cp.ld) ? 0 : ctx->nlao[cp.ld];
         const size_t bytes = sizeof(double) * (size_t)(ncart_c(cp.la) * ncart_c(cp.lb) +
                                                        (ncart_c(cp.la) + ncart_c(cp.lb)) * (cC + cD));
-        const bool fits = bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - kRootTablesMaxBytes);
+        const bool fits = bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - root_table_lds(nroots) - 1024);
         cp.acc_colsC = fits ? cC : 0;
         cp.acc_colsD = fits ? cD : 0;
         cp.wpg = (fits && bytes > kRowAccSmallBytes) ? 8 : 4;
@@ -1058,7 +1060,8 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     if (o.D2 && w.acc_colsC > 0) {  // two accumulator sets must fit the workgroup's LDS
       const size_t nA = ncart_c(w.la), nB = ncart_c(w.lb);
       const size_t bytes = 2 * sizeof(double) * (nA * nB + (nA + nB) * (size_t)(w.acc_colsC + w.acc_colsD));
-      if (bytes > ctx->lds_per_group - kRootTablesMaxBytes) w.acc_colsC = w.acc_colsD = 0;
+      if (bytes > ctx->lds_per_group - root_table_lds((w.la + w.lb + w.lc + w.ld) / 2 + 1) - 1024)
+        w.acc_colsC = w.acc_colsD = 0;
     }
     bool handled = false;
     HIP_CHECK(launch_eri_fast(ctx->dev, w, o, ls, &handled));

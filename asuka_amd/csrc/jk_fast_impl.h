@@ -177,8 +177,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ double fmac(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// Root tables of up to this many roots are copied to LDS by each workgroup.
-constexpr int kLdsRootsMax = 3;
 // Workgroups of up to 8 wavefronts share the root tables and the row
 // accumulators; the host picks 4 or 8 per class pair (ClassWork::wpg): 8 when
 // the accumulators are large (more waves per LDS byte), else 4 (less idle

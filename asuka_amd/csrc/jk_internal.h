@@ -11,6 +11,10 @@
 #define ASUKA_NCART_MAX 10
 #define ASUKA_NROOTS_MAX 7
 
+// The register kernels copy the root tables of up to this many roots into
+// LDS (each workgroup); larger root counts read them from global memory.
+constexpr int kLdsRootsMax = 4;
+
 // One primitive pair of a shell pair (only pairs with a nonzero coefficient
 // product are kept; dropping exact zeros leaves every integral unchanged).
 struct __align__(16) PrimPair {

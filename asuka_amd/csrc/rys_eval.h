@@ -144,6 +144,10 @@ constexpr int lds_stride(int n, int esize) {
   if ((s / per16) % 2 == 0) s += per16;
   return s;
 }
+// LDS bytes of the padded FP64 and FP32 copies of the n-root block
+constexpr size_t lds_table_bytes(int n) {
+  return (size_t)nint[n] * ((size_t)lds_stride(n, 8) * 8 + (size_t)lds_stride(n, 4) * 4);
+}
 static_assert(RYS_H == 2.0, "the block evaluators assume intervals of width 2");
 }  // namespace rys_c
 
