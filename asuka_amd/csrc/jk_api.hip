@@ -26,6 +26,7 @@
 
 hipError_t launch_eri_fast(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s, bool* handled);
 bool eri_fast_has_plan(int la, int lb, int lc, int ld);
+int eri_fast_nch(int la, int lb, int lc, int ld);
 
 static thread_local std::string g_err;
 
@@ -33,6 +34,7 @@ static thread_local std::string g_err;
 static constexpr double kDefaultTauMixed = 1e-7;
 // Row tasks: at most this many wave units of one bra row per workgroup.
 static constexpr int kTaskUnits = 32;
+static constexpr int kTaskItems = 64;  // wave items (unit x chunk) per row task, for split plans
 // LDS accumulators of one row task larger than this (or than the device allows
 // next to the root tables) fall back to global atomics.
 static constexpr size_t kRowAccMaxBytes = 72 * 1024;
@@ -747,6 +749,11 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
         cp.acc_colsD = fits ? cD : 0;
         cp.wpg = (fits && bytes > kRowAccSmallBytes) ? 8 : 4;
         task_units = cp.wpg == 8 ? kTaskUnits : kTaskUnits / 2;
+        // a unit of a plan split into nch component chunks is nch wave items:
+        // keep a row task at about kTaskItems items, so classes with many
+        // chunks get short tasks (a short tail, the queue balances them)
+        const int nch = eri_fast_nch(cp.la, cp.lb, cp.lc, cp.ld);
+        if (nch > 1) task_units = std::max(1, std::min(task_units, kTaskItems / nch));
         cp.task_units = task_units;
       }
       long long padded = 0;

@@ -10,6 +10,26 @@ asuka_fast::Launcher find_launcher_p4(int la, int lb, int lc, int ld);
 asuka_fast::Launcher find_launcher_p5(int la, int lb, int lc, int ld);
 asuka_fast::Launcher find_launcher_p6(int la, int lb, int lc, int ld);
 
+int find_launcher_p0_nch(int la, int lb, int lc, int ld);
+int find_launcher_p1_nch(int la, int lb, int lc, int ld);
+int find_launcher_p2_nch(int la, int lb, int lc, int ld);
+int find_launcher_p3_nch(int la, int lb, int lc, int ld);
+int find_launcher_p4_nch(int la, int lb, int lc, int ld);
+int find_launcher_p5_nch(int la, int lb, int lc, int ld);
+int find_launcher_p6_nch(int la, int lb, int lc, int ld);
+
+// Component chunks of a class's plan (each wave unit is that many wave items); 0 without a plan.
+int eri_fast_nch(int la, int lb, int lc, int ld) {
+  int (*finders[])(int, int, int, int) = {find_launcher_p0_nch, find_launcher_p1_nch, find_launcher_p2_nch,
+                                          find_launcher_p3_nch, find_launcher_p4_nch, find_launcher_p5_nch,
+                                          find_launcher_p6_nch};
+  for (auto fn : finders) {
+    const int n = fn(la, lb, lc, ld);
+    if (n > 0) return n;
+  }
+  return 0;
+}
+
 static asuka_fast::Launcher find_fast(int la, int lb, int lc, int ld) {
   asuka_fast::Launcher f = nullptr;
   asuka_fast::Launcher (*finders[])(int, int, int, int) = {find_launcher_p0, find_launcher_p1, find_launcher_p2, find_launcher_p3, find_launcher_p4, find_launcher_p5, find_launcher_p6};

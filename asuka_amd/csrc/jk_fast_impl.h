@@ -745,12 +745,19 @@ using Launcher = hipError_t (*)(const DevBasis&, const ClassWork&, const EriOut&
 
 }  // namespace asuka_fast
 
-// Each jk_fast_*.hip translation unit instantiates a subset of class plans.
+// Each jk_fast_*.hip translation unit instantiates a subset of class plans;
+// FN returns a plan's launcher, FN##_nch its number of component chunks.
 #define ASUKA_FAST_FIND(FN, LIST)                                                     \
   asuka_fast::Launcher FN(int la, int lb, int lc, int ld) {                          \
     using namespace asuka_fast;                                                       \
     LIST(ASUKA_FAST_CASE)                                                             \
     return nullptr;                                                                   \
+  }                                                                                   \
+  int FN##_nch(int la, int lb, int lc, int ld) {                                      \
+    LIST(ASUKA_FAST_NCH_CASE)                                                         \
+    return 0;                                                                         \
   }
 #define ASUKA_FAST_CASE(a, b_, c, d, sa, sc) \
   if (la == a && lb == b_ && lc == c && ld == d) return &launch_plan<Plan<a, b_, c, d, sa, sc>>;
+#define ASUKA_FAST_NCH_CASE(a, b_, c, d, sa, sc) \
+  if (la == a && lb == b_ && lc == c && ld == d) return (sa) * (sc);
