@@ -34,7 +34,7 @@ static thread_local std::string g_err;
 static constexpr double kDefaultTauMixed = 1e-7;
 // Row tasks: at most this many wave units of one bra row per workgroup.
 static constexpr int kTaskUnits = 32;
-static constexpr int kTaskItems = 64;  // wave items (unit x chunk) per row task, for split plans
+static constexpr int kTaskItems = 256;  // wave items (unit x chunk) per row task, for split plans
 // LDS accumulators of one row task larger than this (or than the device allows
 // next to the root tables) fall back to global atomics.
 static constexpr size_t kRowAccMaxBytes = 72 * 1024;

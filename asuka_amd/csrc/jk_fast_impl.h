@@ -627,9 +627,12 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
     if (t >= w.ntasks) break;  // uniform per workgroup; every workgroup gets here
     const RowTask tk = w.tasks[t];
     const int P = uni(w.units[tk.u0].P);
+    // items chunk-major: the waves of a workgroup run the same chunk's code
+    // over different units (a split plan's chunks are separate code regions,
+    // together far larger than the instruction cache)
     for (int item = wib; item < tk.nu * NCH; item += wpg) {  // uniform per wave
-      const int ui = item / NCH;
-      const int chunk = item - ui * NCH;
+      const int chunk = item / tk.nu;
+      const int ui = item - chunk * tk.nu;
       const WaveUnit un = w.units[tk.u0 + ui];
       bool active = lane < un.ncol;
       const int Qp = w.col_pair[un.col + (active ? lane : 0)];

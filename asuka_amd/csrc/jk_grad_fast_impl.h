@@ -310,8 +310,8 @@ __global__ __launch_bounds__(kThreads) void eri_deriv_fast_kernel(DevBasis b, Cl
     const int t = s_task;
     if (t >= w.ntasks) break;
     const RowTask tk = w.tasks[t];
-    for (int item = wib; item < tk.nu * NCH; item += kWaves) {
-      const int ui = item / NCH, chunk = item - ui * NCH;
+    for (int item = wib; item < tk.nu * NCH; item += kWaves) {  // chunk-major (instruction-cache locality)
+      const int chunk = item / tk.nu, ui = item - chunk * tk.nu;
       const WaveUnit un = w.units[tk.u0 + ui];
       const int P = uni_g(un.P);
       const bool active = lane < un.ncol;
