@@ -91,6 +91,22 @@ __device__ __forceinline__ CT pw(const CT* p, int k) {
   return p[k];
 }
 
+// Plans split over A components whose wave items carry the J[c,d] sums
+// across their A chunks (run_citem). Chosen by measurement at (H2O)8
+// cc-pVTZ: for the other split plans the longer items (fewer per row task)
+// and the extra live registers cost more than the atomics they save.
+constexpr bool carry_jcd(int la, int lb, int lc, int ld) {
+  const int k = ((la * 4 + lb) * 4 + lc) * 4 + ld;
+  constexpr int on[] = {((2 * 4 + 1) * 4 + 1) * 4 + 1, ((2 * 4 + 1) * 4 + 2) * 4 + 1,
+                        ((2 * 4 + 2) * 4 + 1) * 4 + 1, ((2 * 4 + 2) * 4 + 2) * 4 + 0,
+                        ((3 * 4 + 0) * 4 + 2) * 4 + 1, ((3 * 4 + 0) * 4 + 2) * 4 + 2,
+                        ((3 * 4 + 1) * 4 + 2) * 4 + 0, ((3 * 4 + 1) * 4 + 2) * 4 + 1,
+                        ((3 * 4 + 2) * 4 + 2) * 4 + 0, ((3 * 4 + 3) * 4 + 1) * 4 + 0};
+  for (int x : on)
+    if (x == k) return true;
+  return false;
+}
+
 template <int LA, int LB, int LC, int LD, int SA, int SC>
 struct Plan {
   static constexpr int la = LA, lb = LB, lc = LC, ld = LD;
@@ -98,6 +114,7 @@ struct Plan {
   static constexpr int NMAX = LA + LB, MMAX = LC + LD;
   static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
   static constexpr int SPLIT_A = SA, SPLIT_C = SC, NCH = SA * SC;
+  static constexpr bool CARRY = SA > 1 && carry_jcd(LA, LB, LC, LD);
   using RE = Range<LA, LA + LB>;
   using RF = Range<LC, LC + LD>;
   static constexpr int NE = RE::size(), NF = RF::size();
@@ -208,9 +225,13 @@ struct RowAcc {
 // setup, roots, VRR, contracted intermediates and HRR; the digestion always
 // accumulates in FP64. Geometric differences are formed in FP64 and rounded.
 // `tab` is the NR-root block of the root table in CT (LDS or global).
+// jcd: with the plan split over A components, the J[c,d] partial sums of the
+// first density over this chunk's A components are added into jcd (the
+// lane's registers, NCL*ND values) and the caller issues one atomic per
+// element after the last A chunk; nullptr: atomics here.
 template <class PL, int CH, typename CT>
 __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
-                                          const CT* __restrict__ tab, const RowAcc& acc) {
+                                          const CT* __restrict__ tab, const RowAcc& acc, double* jcd) {
   using CK = Chunk<PL, CH>;
   constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
   constexpr int NMAX = PL::NMAX, MMAX = PL::MMAX;
@@ -428,7 +449,10 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
           double s = 0.0;
 #pragma unroll
           for (int r = 0; r < NAL * NB; ++r) s = fma((double)tile[r * NCDL + c], dab[r], s);
-          atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
+          if (jcd && di == 0)
+            jcd[c] = fma(fj, s, jcd[c]);
+          else
+            atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
         }
       }
     }
@@ -527,10 +551,60 @@ __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, con
                                                bool active, const CT* __restrict__ tab, const RowAcc& acc) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH, CT>(b, o, P, Qp, nab, active, tab, acc);
+      run_chunk<PL, CH, CT>(b, o, P, Qp, nab, active, tab, acc, nullptr);
       return;
     }
     dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, nab, active, tab, acc);
+  }
+}
+
+// The A chunks CA.. of C chunk CC, in sequence, carrying J[c,d] in jcd.
+template <class PL, int CC, int CA, typename CT>
+__device__ __forceinline__ void run_agroup(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
+                                           const CT* __restrict__ tab, const RowAcc& acc, double* jcd) {
+  if constexpr (CA < PL::SPLIT_A) {
+    run_chunk<PL, CA * PL::SPLIT_C + CC, CT>(b, o, P, Qp, nab, active, tab, acc, jcd);
+    run_agroup<PL, CC, CA + 1, CT>(b, o, P, Qp, nab, active, tab, acc, jcd);
+  }
+}
+
+// One wave item of a CARRY plan: C chunk CC of a unit, all of its A chunks,
+// with the J[c,d] sums carried across the A chunks in registers, so J[c,d]
+// receives one atomic per element and quartet instead of one per A chunk.
+template <class PL, int CC, typename CT>
+__device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
+                                          const CT* __restrict__ tab, const RowAcc& acc) {
+  using CK = Chunk<PL, CC>;  // A chunk 0: the C range of every A chunk of CC
+  constexpr int ND = PL::ND, NCDL = (CK::c1 - CK::c0) * ND;
+  double jcd[NCDL];
+#pragma unroll
+  for (int c = 0; c < NCDL; ++c) jcd[c] = 0.0;
+  run_agroup<PL, CC, 0, CT>(b, o, P, Qp, nab, active, tab, acc, jcd);
+  double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
+  if (Jo && active && P != Qp) {
+    const int A = uni(b.sp_A[P]), C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
+    long long N, msh;
+    mol_layout(b, A, N, msh);
+    Jo = shifted(Jo, msh);
+    const int aoC = b.shell_ao[C] + CK::c0, aoD = b.shell_ao[Dsh];
+#pragma unroll
+    for (int c = 0; c < NCDL; ++c) atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], jcd[c]);
+  }
+}
+
+// Wave item `item` of a unit: a C chunk with its A chunks for CARRY plans,
+// one chunk otherwise.
+template <class PL, int CC, typename CT>
+__device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const EriOut& o, int P, int Qp, int nab,
+                                               bool active, const CT* __restrict__ tab, const RowAcc& acc) {
+  if constexpr (!PL::CARRY) {
+    dispatch_chunk<PL, 0, CT>(cc, b, o, P, Qp, nab, active, tab, acc);
+  } else if constexpr (CC < PL::SPLIT_C) {
+    if (cc == CC) {
+      run_citem<PL, CC, CT>(b, o, P, Qp, nab, active, tab, acc);
+      return;
+    }
+    dispatch_citem<PL, CC + 1, CT>(cc, b, o, P, Qp, nab, active, tab, acc);
   }
 }
 
@@ -574,7 +648,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
   w.units = units;
   w.col_pair = col_pair;
   o.D = Dm;
-  constexpr int NCH = PL::NCH, NR = PL::NR;
+  constexpr int NIT = PL::CARRY ? PL::SPLIT_C : PL::NCH, NR = PL::NR;  // wave items per unit
   constexpr int NA = nc_(PL::la), NB = PL::NB;
   constexpr bool kLds = NR <= kLdsRootsMax;
   // LDS copies of the root tables, intervals padded (rys_c::lds_stride)
@@ -627,10 +701,12 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
     if (t >= w.ntasks) break;  // uniform per workgroup; every workgroup gets here
     const RowTask tk = w.tasks[t];
     const int P = uni(w.units[tk.u0].P);
-    // items chunk-major: the waves of a workgroup run the same chunk's code
+    // a wave item is one chunk of a unit, or for CARRY plans one C chunk with
+    // its A chunks in sequence; items chunk-major: the waves of a workgroup
+    // run the same chunk's code
     // over different units (a split plan's chunks are separate code regions,
     // together far larger than the instruction cache)
-    for (int item = wib; item < tk.nu * NCH; item += wpg) {  // uniform per wave
+    for (int item = wib; item < tk.nu * NIT; item += wpg) {  // uniform per wave
       const int chunk = item / tk.nu;
       const int ui = item - chunk * tk.nu;
       const WaveUnit un = w.units[tk.u0 + ui];
@@ -669,9 +745,9 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       // bra primitive pairs this unit evaluates (see run_chunk)
       const int nab = uni(o.eps_prim > 0.0 ? un.nab : b.pp_count[P]);
       if (lowp)
-        dispatch_chunk<PL, 0, float>(chunk, b, o, P, Qp, nab, active, tab32, acc);
+        dispatch_citem<PL, 0, float>(chunk, b, o, P, Qp, nab, active, tab32, acc);
       else
-        dispatch_chunk<PL, 0, double>(chunk, b, o, P, Qp, nab, active, tab64, acc);
+        dispatch_citem<PL, 0, double>(chunk, b, o, P, Qp, nab, active, tab64, acc);
     }
     if (use_acc) {
       __syncthreads();  // the task's contributions are in LDS
