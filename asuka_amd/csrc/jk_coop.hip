@@ -25,13 +25,28 @@
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kPackMax = 16;  // registers per lane for packed component offsets
 
 constexpr int ncc(int l) { return (l + 1) * (l + 2) / 2; }
-// Cartesian exponents of l = 0..3, offset l(l+1)(l+2)/6.
-__constant__ signed char k_lx[20] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0, 3, 2, 2, 1, 1, 1, 0, 0, 0, 0};
-__constant__ signed char k_ly[20] = {0, 0, 1, 0, 0, 1, 0, 2, 1, 0, 0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
-__constant__ signed char k_lz[20] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2, 0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
-__constant__ double k_binom[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
+// Cartesian exponents of l = 0..3 (offset l(l+1)(l+2)/6) and the binomial
+// coefficients C(n, m), n, m <= 3, as 2-bit fields of immediates: exponent i
+// at bits 2i, C(n, m) at bits 2(4n + m). Indexed per lane, a shift and a mask
+// replace a memory load.
+constexpr unsigned long long pack2(const signed char* v, int n) {
+  unsigned long long r = 0;
+  for (int i = 0; i < n; ++i) r |= (unsigned long long)v[i] << (2 * i);
+  return r;
+}
+constexpr signed char k_lx[20] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0, 3, 2, 2, 1, 1, 1, 0, 0, 0, 0};
+constexpr signed char k_ly[20] = {0, 0, 1, 0, 0, 1, 0, 2, 1, 0, 0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
+constexpr signed char k_lz[20] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2, 0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+constexpr signed char k_binom[16] = {1, 0, 0, 0, 1, 1, 0, 0, 1, 2, 1, 0, 1, 3, 3, 1};
+constexpr unsigned long long kLX = pack2(k_lx, 20), kLY = pack2(k_ly, 20), kLZ = pack2(k_lz, 20);
+constexpr unsigned long long kBinom = pack2(k_binom, 16);
+static_assert(((kLX >> 20) & 3) == 3 && ((kLY >> 32) & 3) == 3 && ((kLZ >> 38) & 3) == 3 &&
+                  ((kBinom >> 26) & 3) == 3 && ((kBinom >> 18) & 3) == 2,
+              "2-bit tables");
+__device__ __forceinline__ int f2(unsigned long long t, int i) { return (int)((t >> (2 * i)) & 3ull); }
 
 template <int LA, int LB, int LC, int LD>
 struct CoopClass {
@@ -77,6 +92,28 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
   const int lane = threadIdx.x;
   constexpr int oA = LA * (LA + 1) * (LA + 2) / 6, oB = LB * (LB + 1) * (LB + 2) / 6;
   constexpr int oC = LC * (LC + 1) * (LC + 2) / 6, oD = LD * (LD + 1) * (LD + 2) / 6;
+
+  // The lane's components and their 1-D table offsets (ix | iy << 10 | iz << 20)
+  // are the same for every quartet: decoded once, kept in registers for the
+  // classes with at most kPackMax components per lane.
+  constexpr int NPL = (K::NCOMP + kWave - 1) / kWave;
+  constexpr bool kPack = NPL <= kPackMax;
+  static_assert(N1 <= 1024, "1-D table offsets are packed in 10 bits");
+  int pk[kPack ? NPL : 1];
+#pragma unroll
+  for (int j = 0; j < (kPack ? NPL : 1); ++j) {
+    const int comp = lane + kWave * j;
+    int v = 0;
+    if (kPack && comp < K::NCOMP) {
+      const int iab = comp / NCD, icd = comp - iab * NCD;
+      const int ia = iab / NB, ib = iab - ia * NB, ic = icd / ND, id = icd - ic * ND;
+      const int ix = f2(kLX, oA + ia) * SIA + f2(kLX, oB + ib) * SIB + f2(kLX, oC + ic) * SIC + f2(kLX, oD + id);
+      const int iy = f2(kLY, oA + ia) * SIA + f2(kLY, oB + ib) * SIB + f2(kLY, oC + ic) * SIC + f2(kLY, oD + id);
+      const int iz = f2(kLZ, oA + ia) * SIA + f2(kLZ, oB + ib) * SIB + f2(kLZ, oC + ic) * SIC + f2(kLZ, oD + id);
+      v = ix | (iy << 10) | (iz << 20);
+    }
+    pk[j] = v;
+  }
 
   // item k: ket k % 64 of wave unit k / 64 (one quartet per wave at a time)
   for (long long k = blockIdx.x; k < w.nunits * 64; k += gridDim.x) {
@@ -185,21 +222,35 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
           for (int n = ib; n >= 0; --n) {  // sum_n C(ib,n) AB^(ib-n) sum_m C(id,m) CD^(id-m) G[ia+n][ic+m]
             double inner = 0.0, pcd = 1.0;
             for (int m = id; m >= 0; --m) {
-              inner += k_binom[id][m] * pcd * g[(ia + n) * GS + ic + m];
+              inner += (double)f2(kBinom, 4 * id + m) * pcd * g[(ia + n) * GS + ic + m];
               pcd *= CD[ax];
             }
-            val += k_binom[ib][n] * pab * inner;
+            val += (double)f2(kBinom, 4 * ib + n) * pab * inner;
             pab *= AB[ax];
           }
           I[e] = val;
         }
         __syncthreads();
+        if constexpr (kPack) {
+#pragma unroll
+          for (int j = 0; j < NPL; ++j) {  // this lane's components
+            const int comp = lane + kWave * j;
+            if (comp < K::NCOMP) {
+              const int ix = pk[j] & 1023, iy = (pk[j] >> 10) & 1023, iz = pk[j] >> 20;
+              double s = 0.0;
+#pragma unroll
+              for (int uu = 0; uu < NR; ++uu)
+                s += I[(uu * 3) * N1 + ix] * I[(uu * 3 + 1) * N1 + iy] * I[(uu * 3 + 2) * N1 + iz];
+              tile[comp] += s;
+            }
+          }
+        } else
         for (int comp = lane; comp < K::NCOMP; comp += kWave) {  // this lane's components
           const int iab = comp / NCD, icd = comp - iab * NCD;
           const int ia = iab / NB, ib = iab - ia * NB, ic = icd / ND, id = icd - ic * ND;
-          const int ix = k_lx[oA + ia] * SIA + k_lx[oB + ib] * SIB + k_lx[oC + ic] * SIC + k_lx[oD + id];
-          const int iy = k_ly[oA + ia] * SIA + k_ly[oB + ib] * SIB + k_ly[oC + ic] * SIC + k_ly[oD + id];
-          const int iz = k_lz[oA + ia] * SIA + k_lz[oB + ib] * SIB + k_lz[oC + ic] * SIC + k_lz[oD + id];
+          const int ix = f2(kLX, oA + ia) * SIA + f2(kLX, oB + ib) * SIB + f2(kLX, oC + ic) * SIC + f2(kLX, oD + id);
+          const int iy = f2(kLY, oA + ia) * SIA + f2(kLY, oB + ib) * SIB + f2(kLY, oC + ic) * SIC + f2(kLY, oD + id);
+          const int iz = f2(kLZ, oA + ia) * SIA + f2(kLZ, oB + ib) * SIB + f2(kLZ, oC + ic) * SIC + f2(kLZ, oD + id);
           double s = 0.0;
 #pragma unroll
           for (int uu = 0; uu < NR; ++uu)
