@@ -24,11 +24,17 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kNW = 8;  // components per lane per pass
 
-// Cartesian components of l = 0..3 concatenated, offset l(l+1)(l+2)/6.
-__constant__ signed char c_lx[20] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0, 3, 2, 2, 1, 1, 1, 0, 0, 0, 0};
-__constant__ signed char c_ly[20] = {0, 0, 1, 0, 0, 1, 0, 2, 1, 0, 0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
-__constant__ signed char c_lz[20] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2, 0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
-__constant__ double c_binom[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
+// Cartesian exponents of the components of l = 0..3 concatenated (offset
+// l(l+1)(l+2)/6), component i at bits 2i of an immediate.
+constexpr unsigned long long kLX = 0x56B01604ull, kLY = 0x1B18418410ull, kLZ = 0xE491091040ull;
+static_assert(((kLX >> 20) & 3) == 3 && ((kLY >> 32) & 3) == 3 && ((kLZ >> 38) & 3) == 3, "exponents");
+__device__ __forceinline__ int c_exp(unsigned long long t, int i) { return (int)((t >> (2 * i)) & 3ull); }
+
+// Binomial coefficients C(n, m), n, m <= 3, as 2-bit fields of an immediate
+// (bits 2(4n + m)): indexed per lane, a shift and a mask replace a memory load.
+constexpr unsigned int kBinom2 = 0x7D190501u;
+static_assert(((kBinom2 >> 26) & 3) == 3 && ((kBinom2 >> 18) & 3) == 2 && ((kBinom2 >> 30) & 3) == 1, "C(n, m)");
+__device__ __forceinline__ double binom3(int n, int m) { return (double)((kBinom2 >> (2 * (4 * n + m))) & 3u); }
 
 __device__ inline int cart_off(int l) { return l * (l + 1) * (l + 2) / 6; }
 
@@ -131,9 +137,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         if (comp >= ncomp) comp = 0;
         const int iab = comp / nCD, icd = comp - iab * nCD;
         const int ia = iab / nB, ib = iab - ia * nB, ic = icd / nD, id = icd - ic * nD;
-        ix[j] = c_lx[oA + ia] * sIa + c_lx[oB + ib] * sIb + c_lx[oC + ic] * sIc + c_lx[oD + id];
-        iy[j] = c_ly[oA + ia] * sIa + c_ly[oB + ib] * sIb + c_ly[oC + ic] * sIc + c_ly[oD + id];
-        iz[j] = c_lz[oA + ia] * sIa + c_lz[oB + ib] * sIb + c_lz[oC + ic] * sIc + c_lz[oD + id];
+        ix[j] = c_exp(kLX, oA + ia) * sIa + c_exp(kLX, oB + ib) * sIb + c_exp(kLX, oC + ic) * sIc + c_exp(kLX, oD + id);
+        iy[j] = c_exp(kLY, oA + ia) * sIa + c_exp(kLY, oB + ib) * sIb + c_exp(kLY, oC + ic) * sIc + c_exp(kLY, oD + id);
+        iz[j] = c_exp(kLZ, oA + ia) * sIa + c_exp(kLZ, oB + ib) * sIb + c_exp(kLZ, oC + ic) * sIc + c_exp(kLZ, oD + id);
       }
 
       for (int ipq = 0; ipq < nab * ncd; ++ipq) {
@@ -200,8 +206,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             double val = 0.0;
             for (int n = 0; n <= ib_; ++n) {
               double inner = 0.0;
-              for (int m = 0; m <= id_; ++m) inner += c_binom[id_][m] * ipow(cd, id_ - m) * g[(ia_ + n) * s + ic_ + m];
-              val += c_binom[ib_][n] * ipow(ab, ib_ - n) * inner;
+              for (int m = 0; m <= id_; ++m) inner += binom3(id_, m) * ipow(cd, id_ - m) * g[(ia_ + n) * s + ic_ + m];
+              val += binom3(ib_, n) * ipow(ab, ib_ - n) * inner;
             }
             I[e] = val;
           }

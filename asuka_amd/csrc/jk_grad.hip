@@ -41,8 +41,14 @@ constexpr int kWave = 64;
 constexpr int kChunk = 1024;
 constexpr int kGS = 8;  // VRR table stride (indices 0..7 = up to (f f) + 1)
 
-__constant__ double g_binom[5][5] = {
-    {1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
+// Binomial coefficients C(n, m), m <= n <= 4, as 3-bit fields of an immediate
+// at bits 3(n(n+1)/2 + m): indexed per lane, a shift and a mask replace a
+// memory load.
+constexpr unsigned long long kBinom3 = 0x6684B64A249ull;
+static_assert(((kBinom3 >> 36) & 7) == 6 && ((kBinom3 >> 24) & 7) == 3 && ((kBinom3 >> 42) & 7) == 1, "C(n, m)");
+__device__ __forceinline__ double g_binom(int n, int m) {
+  return (double)((kBinom3 >> (3 * (n * (n + 1) / 2 + m))) & 7ull);
+}
 
 // Component i of angular momentum l in the lx-descending, ly-descending order.
 __device__ inline void cart_of(int l, int i, int& x, int& y) {
@@ -203,10 +209,10 @@ __global__ __launch_bounds__(kWave) void eri_deriv_kernel(DevBasis b, ClassWork 
               for (int kk = ib; kk >= 0; --kk) {  // sum_k C(ib,k) AB^(ib-k) ...
                 double inner = 0.0, pcd = 1.0;
                 for (int m = id; m >= 0; --m) {
-                  inner += g_binom[id][m] * pcd * G[(ia + kk) * kGS + ic + m];
+                  inner += g_binom(id, m) * pcd * G[(ia + kk) * kGS + ic + m];
                   pcd *= CD[ax];
                 }
-                acc += g_binom[ib][kk] * pab * inner;
+                acc += g_binom(ib, kk) * pab * inner;
                 pab *= AB[ax];
               }
               s_I[ax * LY.nI + r] = acc;
