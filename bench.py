@@ -34,7 +34,7 @@ METRIC = "Fock-build s & quartets/s: mixed vs FP64 GPU at ≤1e-6 Eh; gradient s
 # HBM bytes of one build's ERI kernels (FETCH_SIZE + WRITE_SIZE, each scaled
 # by what the counter reads for a 1 GiB device copy), measured with rocprofv3
 # --pmc on the FP64 build: profiles/jk_build_h2o8_ccpvtz_hbm_traffic.md.
-HBM_TRAFFIC = {2: 1.711e10}
+HBM_TRAFFIC = {2: 1.877e10}
 
 CONFIGS = {
     2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12,
