@@ -75,6 +75,9 @@ def lib():
         L.oracle_eri_deriv.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        _f64p, _f64p]
         L.oracle_eri_deriv.restype = ctypes.c_int
+        _i64p = ctypes.POINTER(ctypes.c_int64)
+        L.oracle_screen_counts.argtypes = [ctypes.c_long, _i64p, ctypes.c_long, _f64p, _f64p, ctypes.c_double,
+                                           ctypes.c_double, _i32p, _i32p, _i64p, ctypes.c_int, ctypes.c_int, _i64p]
         L.oracle_grad2e.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, _f64p, _f64p, _f64p, _i32p,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
         _LIB = L
@@ -243,6 +246,63 @@ def density_screen(ctx, Q, D, ab, cd, eps_density):
     val = Q[ab] * Q[cd] * np.maximum(dsp[ab], dsp[cd])
     keep = val >= eps_density
     return ab[keep], cd[keep]
+
+
+def pair_class(la: int, lb: int) -> int:
+    """Pair class of an l-ordered shell pair (la >= lb): la(la+1)/2 + lb."""
+    return la * (la + 1) // 2 + lb
+
+
+def nonzero_prim_pairs(ctx: OracleContext) -> np.ndarray:
+    """Primitive pairs per shell pair whose coefficient product cK is not an
+    exact zero (the K of the work model, hf/direct_jk_cost.py)."""
+    nz = ctx.pair_cK != 0.0
+    out = np.zeros(ctx.nsp, dtype=np.int64)
+    for sp in range(ctx.nsp):
+        s0, n = int(ctx.sp_pair_start[sp]), int(ctx.sp_npair[sp])
+        out[sp] = int(nz[s0:s0 + n].sum())
+    return out
+
+
+def screen_counts(ctx: OracleContext, Q: np.ndarray, Dsp: np.ndarray | None, eps_schwarz: float,
+                  eps_density: float, nprim: np.ndarray | None = None, threads: int = 0) -> dict:
+    """Kept canonical quartets per (bra class, ket class) pair and symmetry bin,
+    without materialising the list: the Q-descending enumeration of
+    build_tasks_screened_sorted_q (cueri/tasks.py:49-115, hf/direct_jk.py:
+    1228-1246) and the CSAM mask of hf/direct_jk.py:1893-1909 (Dsp None: Schwarz
+    only). Returns {(la, lb, lc, ld): int64[9]}: bins
+    (A!=B)<<2 | (C!=D)<<1 | (bra != ket) with the bra the pair of the higher
+    class (the later rank inside one class), and [8] the primitive quartets."""
+    b = ctx.basis
+    L = np.asarray(b.shell_l)
+    la = L[ctx.sp_A].astype(np.int64)
+    lb = L[ctx.sp_B].astype(np.int64)
+    cls = np.ascontiguousarray(la * (la + 1) // 2 + lb, dtype=np.int32)
+    neq = np.ascontiguousarray(ctx.sp_A != ctx.sp_B, dtype=np.int32)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    perm = np.ascontiguousarray(np.argsort(-Q, kind="stable"), dtype=np.int64)
+    n_valid = int(np.searchsorted(-Q[perm], 0.0, side="left"))
+    if nprim is None:
+        nprim = nonzero_prim_pairs(ctx)
+    nprim = np.ascontiguousarray(nprim, dtype=np.int64)
+    lmax = int(L.max())
+    ncls = pair_class(lmax, lmax) + 1
+    out = np.zeros((ncls, ncls, 9), dtype=np.int64)
+    dsp = None if Dsp is None else np.ascontiguousarray(Dsp, dtype=np.float64)
+    _i64p = ctypes.POINTER(ctypes.c_int64)
+    lib().oracle_screen_counts(ctypes.c_long(ctx.nsp), _p(perm, _i64p), ctypes.c_long(n_valid), _p(Q, _f64p),
+                               _p(dsp, _f64p), float(eps_schwarz), float(eps_density), _p(cls, _i32p),
+                               _p(neq, _i32p), _p(nprim, _i64p), ncls, int(threads), _p(out, _i64p))
+    names = {}
+    for l1 in range(lmax + 1):
+        for l2 in range(l1 + 1):
+            names[pair_class(l1, l2)] = (l1, l2)
+    res = {}
+    for X in range(ncls):
+        for Y in range(X + 1):
+            if out[X, Y].any():
+                res[names[X] + names[Y]] = out[X, Y].copy()
+    return res
 
 
 def direct_jk_tasks(ctx: OracleContext, D: np.ndarray, ab: np.ndarray, cd: np.ndarray, hp: bool = True,
