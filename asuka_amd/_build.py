@@ -52,13 +52,41 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None) -> str:
-    """Compile every HIP source and link the shared library; returns its path."""
+def _local_includes(path: str, seen: set) -> None:
+    """Adds path and the files its #include "..." lines name, recursively."""
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            t = line.strip()
+            if t.startswith("#include") and '"' in t:
+                _local_includes(os.path.join(os.path.dirname(path), t.split('"')[1]), seen)
+
+
+def _stale(src: str) -> bool:
+    """An object is stale when it is older than its source or any header it
+    includes (directly or not) from this tree."""
+    obj = os.path.join(OBJDIR, os.path.basename(src)[:-4] + ".o")
+    if not os.path.exists(obj):
+        return True
+    deps: set = set()
+    _local_includes(src, deps)
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile every HIP source (with incremental=True only those whose object
+    is stale) and link the shared library; returns its path."""
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
-    jobs = jobs or min(len(srcs), os.cpu_count() or 1, 16)
+    todo = [s for s in srcs if _stale(s)] if incremental else srcs
+    jobs = jobs or min(max(1, len(todo)), os.cpu_count() or 1, 16)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJDIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -69,4 +97,6 @@ def build(jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    print(build())
+    import sys
+
+    print(build(incremental="-i" in sys.argv[1:]))

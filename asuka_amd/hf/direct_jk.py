@@ -91,6 +91,21 @@ class DirectJKContext:
                         "prim_quartets": int(r[5]), "prim_roots": int(r[6])})
         return out
 
+    def last_counts(self) -> list[dict]:
+        """Counters of the last build run with the count flag (``stats=`` of
+        :func:`direct_JK`): per class pair, in :meth:`class_stats` order, the
+        quartets evaluated after density screening by symmetry bin
+        (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair), their primitive
+        quartets, and the FP32 share of both."""
+        cls = self.class_stats()
+        n = len(cls)
+        rows = np.zeros((max(1, n), 16), dtype=np.int64)
+        _lib.check(_lib.lib().asuka_jk_last_counts(ctypes.c_void_p(self.handle), rows.ctypes.data_as(_lib._i64p), n,
+                                                   ctypes.c_void_p(_lib.stream_handle(self.device))))
+        return [{"class": c["class"], "bins": rows[i, :8].copy(), "prim_quartets": int(rows[i, 8]),
+                 "quartets_fp32": int(rows[i, 9]), "prim_quartets_fp32": int(rows[i, 10])}
+                for i, c in enumerate(cls)]
+
     def stats(self) -> dict:
         """Work summary (asuka_jk_stats): Schwarz-screened quartets and, from the
         last build with the count flag, the quartets and primitive quartets
@@ -232,8 +247,11 @@ def _as_device_matrix(torch, X, nao: int, name: str, device):
 
 
 def _flags(mixed_precision: bool | None) -> int:
+    """Precision flag of a build. None takes the reference's default, mixed
+    precision unless ``ASUKA_ERI_MIXED_PRECISION=0`` (asuka/hf/direct_jk.py:
+    1580-1581)."""
     if mixed_precision is None:
-        mixed_precision = False
+        mixed_precision = os.environ.get("ASUKA_ERI_MIXED_PRECISION", "1") != "0"
     return _lib.MIXED if mixed_precision else 0
 
 
@@ -269,6 +287,8 @@ def direct_JK(
         J = torch.empty((nao, nao), dtype=torch.float64, device=ctx.device) if want_J else None
         K = torch.empty((nao, nao), dtype=torch.float64, device=ctx.device) if want_K else None
     flags = (_lib.WANT_J if want_J else 0) | (_lib.WANT_K if want_K else 0) | _flags(mixed_precision)
+    if stats is not None:
+        flags |= _lib.COUNT
     t0 = time.perf_counter()
     _lib.check(_lib.lib().asuka_jk_build(ctypes.c_void_p(ctx.handle), _lib.ptr(Dd), _lib.ptr(J if want_J else None),
                                          _lib.ptr(K if want_K else None), flags,
@@ -278,9 +298,18 @@ def direct_JK(
         profile["direct_jk_t_s"] = float(time.perf_counter() - t0)
         profile["direct_jk_ntasks"] = int(ctx.ntasks)
     if stats is not None:
-        stats["direct_jk_ntasks"] = int(ctx.ntasks)
-        stats["classes"] = ctx.class_stats()
+        _fill_stats(ctx, stats)
     return (J if want_J else None), (K if want_K else None)
+
+
+def _fill_stats(ctx: DirectJKContext, stats: dict) -> None:
+    """The stats dict of a counted build (asuka/hf/direct_jk.py:1687-1700):
+    Schwarz-screened and evaluated quartets, per class pair."""
+    counts = ctx.last_counts()
+    stats["direct_jk_ntasks"] = int(ctx.ntasks)
+    stats["classes"] = ctx.class_stats()
+    stats["class_counts"] = counts
+    stats["direct_jk_evaluated"] = int(sum(int(c["bins"].sum()) for c in counts))
 
 
 def direct_JK_multi(
@@ -361,15 +390,15 @@ def direct_fock_rhf(
     hd = _as_device_matrix(torch, hcore, nao, "hcore", ctx.device)
     F = out if out is not None else torch.empty((nao, nao), dtype=torch.float64, device=ctx.device)
     t0 = time.perf_counter()
+    flags = _flags(mixed_precision) | (_lib.COUNT if stats is not None else 0)
     _lib.check(_lib.lib().asuka_fock_build(ctypes.c_void_p(ctx.handle), _lib.ptr(Dd), _lib.ptr(hd), _lib.ptr(F),
-                                           _flags(mixed_precision), ctypes.c_void_p(_lib.stream_handle(ctx.device))))
+                                           flags, ctypes.c_void_p(_lib.stream_handle(ctx.device))))
     if profile is not None:
         torch.cuda.synchronize(ctx.device)
         profile["direct_fock_t_s"] = float(time.perf_counter() - t0)
         profile["direct_fock_ntasks"] = int(ctx.ntasks)
     if stats is not None:
-        stats["direct_jk_ntasks"] = int(ctx.ntasks)
-        stats["classes"] = ctx.class_stats()
+        _fill_stats(ctx, stats)
     return F
 
 

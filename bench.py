@@ -1,17 +1,22 @@
 #!/usr/bin/env python
 """Benchmark of the integral-direct J/K engine on one MI355X.
 
-Default workload: (H2O)8 water cluster (2x2x2 grid, 3.0 A, seed 2), cc-pVTZ
-cartesian (520 AOs), D = C C^T with C = QR(N(0,1) 520 x 40) (seed 1002), one
-direct J/K build per step at Schwarz 1e-12 and density screening 1e-10.
+Default workload: (H2O)64 water cluster (4x4x4 grid, 3.0 A, seed 5), cc-pVDZ
+cartesian (1600 AOs), D = 2 C C^T with C = QR(N(0,1) 1600 x 320) (seed 1005),
+one direct RHF Fock build F = h + J - K/2 per step at Schwarz 1e-12 and density
+screening 1e-10, timed with the all-FP64 build and with the mixed FP32/FP64
+build. --config 2 times one J/K build of (H2O)8 cc-pVTZ (520 AOs); --config 3
+the batched build of 32 clusters; --config 4 the (H2O)16 cc-pVDZ gradient;
+--config 5 --scf 15 RHF iterations.
 
-Prints one JSON line: shell quartets/s (value), seconds per build, EFM-v1
-roofline of the ERI/digestion kernels (HIP events on the launch stream), a
-parity stamp against the CPU oracle on sampled tiles, and the CPU baseline (the
-oracle's reference-arithmetic build on a bounded sample of the same screened
-quartets, timed on this host's cores).
+Prints one JSON line: shell quartets/s of the mixed build (value), seconds per
+build in both modes and their accuracy (max |dF|, max |dJ|, max |dK|, dE), the
+EFM-v1 roofline of the ERI/digestion kernels (HIP events on the launch
+stream), a parity stamp against the CPU oracle on sampled tiles, and the CPU
+baseline (the oracle's reference-arithmetic build on a bounded sample of the
+same screened quartets, timed on this host's cores).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fp64|mixed]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2|3|4|5] [--mode fp64|mixed|both]
 """
 
 from __future__ import annotations
@@ -33,14 +38,15 @@ METRIC = "Fock-build s & quartets/s: mixed vs FP64 GPU at ≤1e-6 Eh; gradient s
 
 # HBM bytes of one build's ERI kernels (FETCH_SIZE + WRITE_SIZE, each scaled
 # by what the counter reads for a 1 GiB device copy), measured with rocprofv3
-# --pmc on the FP64 build: profiles/jk_build_h2o8_ccpvtz_hbm_traffic.md.
-HBM_TRAFFIC = {2: 1.877e10}
+# --pmc per workload and mode (profiles/*_hbm_traffic.md); None where not
+# measured.
+HBM_TRAFFIC = {(2, "fp64"): 1.877e10}
 
 CONFIGS = {
-    2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12,
+    2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12, kind="jk",
             workload="(H2O)8 cc-pVTZ cart, one direct J/K build"),
-    5: dict(n=64, seed=5, basis="cc-pVDZ", nocc=320, cseed=1005, eps=1e-12,
-            workload="(H2O)64 cc-pVDZ cart, one direct Fock build",
+    5: dict(n=64, seed=5, basis="cc-pVDZ", nocc=320, cseed=1005, eps=1e-12, kind="fock",
+            workload="(H2O)64 cc-pVDZ cart, one direct RHF Fock build",
             workload_scf="(H2O)64 cc-pVDZ cart, 15 direct RHF iterations"),
     3: dict(nmol=32, seed=3, basis="cc-pVDZ", eps=1e-12,
             workload="32 water clusters of 1-16 molecules, cc-pVDZ cart, one batched J/K build"),
@@ -161,8 +167,11 @@ def run_batched(args, cfg) -> int:
 def run_scf(args, cfg) -> int:
     """Large direct SCF: (H2O)64 cc-pVDZ (1600 AOs), 15 RHF iterations (core
     guess, one predictor Fock build, DIIS) with the all-FP64 build and with the
-    mixed FP32/FP64 build, on one context; the energy after 15 iterations of
-    each is compared."""
+    mixed FP32/FP64 build, on one context, each SCF iteration one replay of a
+    captured HIP graph (Fock build, device DIIS, purified density, energy and
+    convergence test); the energy after 15 iterations of each is compared. The
+    time outside the Fock builds is the SCF time minus that of the same number
+    of Fock builds alone."""
     import torch
 
     from asuka_amd.frontend import build_int1e
@@ -184,24 +193,40 @@ def run_scf(args, cfg) -> int:
     out = {}
     kw = dict(nelec=mol.nelectron, enuc=float(mol.energy_nuc()))
     D15 = None
+    from asuka_amd.hf.direct_jk import direct_fock_rhf
+
+    hd = ints.hcore if isinstance(ints.hcore, torch.Tensor) else torch.as_tensor(np.asarray(ints.hcore), device=dev)
     for mode in ("fp64", "mixed"):
+        mixed = mode == "mixed"
+        # warm-up: one short SCF (graph capture, allocator pools)
+        rhf_direct(ints.S, ints.hcore, ctx, max_cycle=2, conv_tol=0.0, conv_tol_dm=0.0, mixed_precision=mixed, **kw)
         torch.cuda.synchronize(dev)
+        prof = {}
         t0 = time.perf_counter()
         r = rhf_direct(ints.S, ints.hcore, ctx, max_cycle=niter, conv_tol=0.0, conv_tol_dm=0.0,
-                       mixed_precision=(mode == "mixed"), **kw)
+                       mixed_precision=mixed, profile=prof, **kw)
         torch.cuda.synchronize(dev)
-        out[mode] = {"s": time.perf_counter() - t0, "e_tot": r.e_tot, "iters": r.niter}
+        t_scf = time.perf_counter() - t0
+        nfock = int(prof["scf"]["jk_calls"]) + 1  # + the predictor cycle
+        F = torch.empty_like(hd)
+        t0 = time.perf_counter()
+        for _ in range(nfock):
+            direct_fock_rhf(ctx, r.dm, hd, mixed_precision=mixed, out=F)
+        torch.cuda.synchronize(dev)
+        t_fock = time.perf_counter() - t0
+        out[mode] = {"s": t_scf, "e_tot": r.e_tot, "iters": r.niter, "fock_builds": nfock, "fock_s": t_fock,
+                     "non_fock_fraction": (t_scf - t_fock) / t_scf, "graph": bool(prof["scf"].get("graph")),
+                     "purify_steps": prof["scf"].get("purify_steps"),
+                     "purify_err_max": prof["scf"].get("purify_err_max")}
         if mode == "fp64":
             D15 = r.dm
-        log(f"{mode}: {niter} SCF iterations in {out[mode]['s']:.2f} s, E = {r.e_tot:.10f}")
+        log(f"{mode}: {niter} SCF iterations in {t_scf:.2f} s ({nfock} Fock builds alone {t_fock:.2f} s, "
+            f"outside the builds {100 * (t_scf - t_fock) / t_scf:.1f} %), E = {r.e_tot:.10f}")
     dE = out["mixed"]["e_tot"] - out["fp64"]["e_tot"]
     # precision of one build at the FP64 run's iteration-15 density:
     # E = 1/2 sum D (h + F), so dE = 1/2 sum D (F_mixed - F_fp64)
-    from asuka_amd.hf.direct_jk import direct_fock_rhf
-
-    h = torch.as_tensor(np.asarray(ints.hcore), device=dev) if not isinstance(ints.hcore, torch.Tensor) else ints.hcore
-    F64 = direct_fock_rhf(ctx, D15, h)
-    F32 = direct_fock_rhf(ctx, D15, h, mixed_precision=True)
+    F64 = direct_fock_rhf(ctx, D15, hd, mixed_precision=False)
+    F32 = direct_fock_rhf(ctx, D15, hd, mixed_precision=True)
     dE_build = float(0.5 * (D15 * (F32 - F64)).sum())
     # converged energies, both modes continued from that density
     conv = {}
@@ -229,6 +254,7 @@ def run_scf(args, cfg) -> int:
         "fp64_s": out["fp64"]["s"],
         "mixed_s": out["mixed"]["s"],
         "speedup_mixed_vs_fp64": out["fp64"]["s"] / out["mixed"]["s"],
+        "per_mode": out,
         "e_tot_fp64": out["fp64"]["e_tot"],
         "e_tot_mixed": out["mixed"]["e_tot"],
         "dE_mixed_vs_fp64_after_15_iterations_Eh": dE,
@@ -256,7 +282,7 @@ def run_gradient(args, cfg) -> int:
     torch.cuda.set_device(dev)
     mol = water_cluster(cfg["n"], cfg["seed"], cfg["basis"])
     t0 = time.perf_counter()
-    res = run_hf_df(mol, two_e_backend="direct", eps_schwarz=cfg["eps"], conv_tol=1e-10)
+    res = run_hf_df(mol, two_e_backend="direct", eps_schwarz=cfg["eps"], conv_tol=1e-10, mixed_precision=False)
     t_scf = time.perf_counter() - t0
     ctx = res.direct_jk_ctx
     log(f"SCF: nao={ctx.nao} E={res.scf.e_tot:.10f} converged={res.scf.converged} in {t_scf:.1f}s")
@@ -298,7 +324,7 @@ def run_gradient(args, cfg) -> int:
         # is tightened (Schwarz 1e-14, no density screening) for the check, so
         # the screened-away set does not jump between displaced geometries.
         os.environ["ASUKA_DIRECT_JK_EPS_DENSITY"] = "0"
-        kw = dict(two_e_backend="direct", eps_schwarz=1e-14, conv_tol=1e-12, conv_tol_dm=1e-9)
+        kw = dict(two_e_backend="direct", eps_schwarz=1e-14, conv_tol=1e-12, conv_tol_dm=1e-9, mixed_precision=False)
         rt = run_hf_df(mol, dm0=res.scf.dm, **kw)
         gt = rhf_nuc_grad(rt)
         h = 1e-4
@@ -389,7 +415,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--config", type=int, default=5, choices=sorted(CONFIGS))
     ap.add_argument("--mode", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
@@ -437,13 +463,23 @@ def main() -> int:
     nao = ao.nao
     C = np.random.default_rng(cfg["cseed"]).standard_normal((nao, cfg["nocc"]))
     C, _ = np.linalg.qr(C)
-    D_h = C @ C.T
+    fock = cfg["kind"] == "fock"
+    D_h = (2.0 if fock else 1.0) * (C @ C.T)
     t0 = time.perf_counter()
-    ctx = make_direct_jk_context(ao, eps_schwarz=cfg["eps"], tau_mixed=args.tau)
+    ctx = make_direct_jk_context(ao, eps_schwarz=cfg["eps"], tau_mixed=args.tau, atom_coords_bohr=mol.coords_bohr,
+                                 atom_charges=mol.charges)
     t_ctx = time.perf_counter() - t0
     log(f"context: nao={nao} nsp={ctx.nsp} schwarz-screened quartets={ctx.ntasks} ({t_ctx:.2f}s)")
 
     D = torch.as_tensor(D_h, device=dev)
+    hcore = None
+    if fock:
+        from asuka_amd.frontend import build_int1e
+
+        hcore = build_int1e(ctx).hcore
+        if not isinstance(hcore, torch.Tensor):
+            hcore = torch.as_tensor(np.asarray(hcore), device=dev)
+        hcore = hcore.to(device=dev, dtype=torch.float64).contiguous()
     t_tune = None
     if args.autotune:
         t0 = time.perf_counter()
@@ -458,10 +494,15 @@ def main() -> int:
     def run_mode(mode: str):
         J = torch.empty((nao, nao), dtype=torch.float64, device=dev)
         K = torch.empty_like(J)
+        F = torch.empty_like(J)
         flags = _lib.WANT_J | _lib.WANT_K | (_lib.MIXED if mode == "mixed" else 0)
 
         def build(extra=0):
-            _lib.check(L.asuka_jk_build(h, _lib.ptr(D), _lib.ptr(J), _lib.ptr(K), flags | extra, sptr))
+            if fock:
+                _lib.check(L.asuka_fock_build(h, _lib.ptr(D), _lib.ptr(hcore), _lib.ptr(F),
+                                              (flags & _lib.MIXED) | extra, sptr))
+            else:
+                _lib.check(L.asuka_jk_build(h, _lib.ptr(D), _lib.ptr(J), _lib.ptr(K), flags | extra, sptr))
 
         for _ in range(max(0, args.warmup)):
             build()
@@ -486,8 +527,11 @@ def main() -> int:
             kern_ms.append(ms.value)
         torch.cuda.synchronize(dev)
         wall = time.perf_counter() - t0
-        r = {"J": J, "K": K, "work": work, "f32": f32, "q32": q32, "s_per_build": wall / args.steps,
-             "kern_s": float(np.mean(kern_ms)) / 1e3, "nlaunch": int(nl.value)}
+        if fock:  # J and K of the same mode for the accuracy stamp (untimed)
+            _lib.check(L.asuka_jk_build(h, _lib.ptr(D), _lib.ptr(J), _lib.ptr(K), flags, sptr))
+            torch.cuda.synchronize(dev)
+        r = {"J": J, "K": K, "F": F if fock else None, "work": work, "f32": f32, "q32": q32,
+             "s_per_build": wall / args.steps, "kern_s": float(np.mean(kern_ms)) / 1e3, "nlaunch": int(nl.value)}
         log(f"{mode}: {r['s_per_build'] * 1e3:.2f} ms/build, ERI kernels {r['kern_s'] * 1e3:.2f} ms over "
             f"{r['nlaunch']} launches, quartets={work.quartets} (FP32 {q32}), EFM-v1 GFLOP={work.flops / 1e9:.2f}")
         return r
@@ -507,9 +551,14 @@ def main() -> int:
     if "fp64" in res and "mixed" in res:
         dJ = res["mixed"]["J"] - res["fp64"]["J"]
         dK = res["mixed"]["K"] - res["fp64"]["K"]
-        dE = float(0.5 * (D * dJ).sum() - 0.25 * (D * dK).sum())
+        if fock:  # E = 1/2 sum D (h + F) at fixed D
+            dF = res["mixed"]["F"] - res["fp64"]["F"]
+            dE = float(0.5 * (D * dF).sum())
+        else:  # E_2e of the J/K build's density (D = C C^T, closed-shell form)
+            dE = float(0.5 * (D * dJ).sum() - 0.25 * (D * dK).sum())
         accuracy = {"max_abs_dJ": float(dJ.abs().max()), "max_abs_dK": float(dK.abs().max()),
-                    "dE_2e_Eh": dE, "tau_mixed": args.tau,
+                    "max_abs_dF": float(dF.abs().max()) if fock else None,
+                    "dE_Eh": dE, "tau_mixed": args.tau,
                     "fp64_s_per_build": res["fp64"]["s_per_build"],
                     "mixed_s_per_build": res["mixed"]["s_per_build"],
                     "speedup_mixed_vs_fp64": res["fp64"]["s_per_build"] / res["mixed"]["s_per_build"],
@@ -577,7 +626,8 @@ def main() -> int:
         "autotune_s": t_tune,
         "mixed_vs_fp64": accuracy,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": metrics.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": metrics.roofline_fraction(f64, f32, kern_s), "traffic": HBM_TRAFFIC.get(args.config),
+                     "frac": metrics.roofline_fraction(f64, f32, kern_s),
+                     "traffic": HBM_TRAFFIC.get((args.config, main_mode)),
                      "flops_fp64": f64, "flops_fp32": f32,
                      "flops_per_build_efm_v1": work.flops, "kernel_ms_per_build": kern_s * 1e3,
                      "kernel_launches_per_build": int(nl.value),

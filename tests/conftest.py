@@ -11,6 +11,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 if HERE not in sys.path:
     sys.path.insert(0, HERE)
 
+# FP64 parity: builds without an explicit precision run in FP64 (the reference's
+# own FP64 parity runs set the same variable, its default being mixed)
+os.environ["ASUKA_ERI_MIXED_PRECISION"] = "0"
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU and the built HIP library")

@@ -230,17 +230,20 @@ def test_config2_size_properties(gpu):
     assert float(ev.min()) > -1e-9 * float(ev.max())
 
 
-def test_mixed_precision_jk(gpu, jk_golden):
+@pytest.mark.parametrize("case", ["h2o4_dz", "h2o2_tz"])
+def test_mixed_precision_jk(gpu, jk_golden, case):
     """Mixed FP32/FP64 mode. With the default bound cut (1e-7) J and K stay
     within 1e-8 of the FP64 oracle; with every quartet forced onto the FP32
     kernels (cut 1e3) they stay within 1e-5, which no index or layout mistake
     in the FP32 instantiations would meet, and differ from FP64 (so the FP32
-    path did run)."""
+    path did run). The cc-pVTZ case runs the FP32 instantiations of the f-shell
+    plans."""
     from asuka_amd.hf.direct_jk import direct_JK
 
-    mol = water_cluster(4, 4, "cc-pVDZ")
-    D = jk_golden["h2o4_dz_D"]
-    Jr, Kr = jk_golden["h2o4_dz_J_scr"], jk_golden["h2o4_dz_K_scr"]
+    n, seed, basis = {"h2o4_dz": (4, 4, "cc-pVDZ"), "h2o2_tz": (2, 2, "cc-pVTZ")}[case]
+    mol = water_cluster(n, seed, basis)
+    D = jk_golden[f"{case}_D"]
+    Jr, Kr = jk_golden[f"{case}_J_scr"], jk_golden[f"{case}_K_scr"]
     ao, ctx = _ctx(mol, eps_schwarz=1e-12)
     J, K = direct_JK(ctx, D, mixed_precision=True)
     assert np.max(np.abs(J.cpu().numpy() - Jr)) < 1e-8
