@@ -18,8 +18,11 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-LIB = os.path.join(HERE, "libasuka_jk.so")
-OBJDIR = os.path.join(HERE, "_objs")
+# ASUKA_BUILD_DIR puts the objects and the library of a variant build
+# elsewhere (with extra hipcc flags from ASUKA_BUILD_FLAGS); default in-tree
+_OUT = os.environ.get("ASUKA_BUILD_DIR")
+LIB = os.path.join(_OUT, "libasuka_jk.so") if _OUT else os.path.join(HERE, "libasuka_jk.so")
+OBJDIR = os.path.join(_OUT, "_objs") if _OUT else os.path.join(HERE, "_objs")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -45,7 +48,8 @@ def sources() -> list[str]:
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src)[:-4] + ".o")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", CSRC, "-c", src, "-o", obj]
+    extra = os.environ.get("ASUKA_BUILD_FLAGS", "").split()
+    cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-I", CSRC, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

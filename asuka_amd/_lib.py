@@ -13,7 +13,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libasuka_jk.so")
+# ASUKA_JK_LIB points at another build of the same library (A/B timing of
+# kernel variants); the in-tree build is the default
+LIB_PATH = os.environ.get("ASUKA_JK_LIB") or os.path.join(_HERE, "libasuka_jk.so")
 
 ASUKA_OK = 0
 ASUKA_E_INVALID = -1
@@ -85,6 +87,12 @@ SYMBOLS = [
     ("asuka_jk_ctx_destroy", ctypes.c_int, [_vp]),
     ("asuka_jk_ctx_info", ctypes.c_int, [_vp, _i64p]),
     ("asuka_jk_build", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    ("asuka_scf_diis_solve", ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    ("asuka_df_ctx_create", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("asuka_df_ctx_destroy", ctypes.c_int, [_vp]),
+    ("asuka_df_ctx_info", ctypes.c_int, [_vp, _vp]),
+    ("asuka_df_metric", ctypes.c_int, [_vp, _vp, _vp]),
+    ("asuka_df_int3c2e", ctypes.c_int, [_vp, _vp, _vp]),
     ("asuka_jk_build_multi", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_fock_build", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_jk_autotune", ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),

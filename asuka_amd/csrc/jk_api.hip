@@ -59,6 +59,9 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+// the same for the other translation units of the library
+int asuka_fail_msg(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+
 #define HIP_CHECK(expr)                                                                       \
   do {                                                                                        \
     hipError_t _e = (expr);                                                                   \
@@ -674,6 +677,16 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
   }
   for (auto& v : cls_pairs)
     std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return ctx->Q[a] > ctx->Q[b]; });
+  // rank of each shell pair in the stable Q-descending order of all pairs
+  // (asuka/hf/direct_jk.py:1228 argsort(-Q, kind="stable")); inside a class
+  // list the ranks increase
+  std::vector<int> rank(ctx->nsp);
+  {
+    std::vector<int> ord(ctx->nsp);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ctx->Q[a] > ctx->Q[b]; });
+    for (int r = 0; r < ctx->nsp; ++r) rank[ord[r]] = r;
+  }
   std::vector<int> cls_la(ncls), cls_lb(ncls);
   for (int la = 0; la <= ASUKA_LMAX; ++la)
     for (int lb = 0; lb <= la; ++lb) {
@@ -749,9 +762,9 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
         cp.acc_colsD = fits ? cD : 0;
         cp.wpg = (fits && bytes > kRowAccSmallBytes) ? 8 : 4;
         task_units = cp.wpg == 8 ? kTaskUnits : kTaskUnits / 2;
-        // a unit of a plan split into nch component chunks is nch wave items:
-        // keep a row task at about kTaskItems items, so classes with many
-        // chunks get short tasks (a short tail, the queue balances them)
+        // a unit of a split plan is nch wave items: keep a row task at about
+        // kTaskItems items, so classes with many chunks get short tasks (a
+        // short tail, the queue balances them)
         const int nch = eri_fast_nch(cp.la, cp.lb, cp.lc, cp.ld);
         if (nch > 1) task_units = std::max(1, std::min(task_units, kTaskItems / nch));
         cp.task_units = task_units;
@@ -768,9 +781,22 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
             const int P = bg.pairs[r];
             long long len;
             if (eps > 0.0) {
+              // the reference's rule, Q_j >= eps / Q_i for the pair i of the
+              // later rank in the global Q-descending order; the kets ranked
+              // before P come first in cols
               const double thr = eps / std::max(ctx->Q[P], 1e-300);
-              len = std::partition_point(cols.begin(), cols.end(), [&](int c) { return ctx->Q[c] >= thr; }) -
-                    cols.begin();
+              auto keep_before = [&](int c) { return ctx->Q[c] >= thr; };
+              if (X == Y) {
+                len = std::partition_point(cols.begin(), cols.end(), keep_before) - cols.begin();
+              } else {
+                const auto split = std::partition_point(cols.begin(), cols.end(),
+                                                        [&](int c) { return rank[c] < rank[P]; });
+                len = std::partition_point(cols.begin(), split, keep_before) - cols.begin();
+                if (cols.begin() + len == split)
+                  len = std::partition_point(split, cols.end(), [&](int c) {
+                          return ctx->Q[P] >= eps / std::max(ctx->Q[c], 1e-300);
+                        }) - cols.begin();
+              }
             } else {
               len = (long long)cols.size();
             }
@@ -860,7 +886,8 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
   {
     std::vector<unsigned long long> cz(kCountStride * std::max<size_t>(1, ctx->cps.size()), 0ull);
     if ((rc = dev_upload(&ctx->d_counters, cz, own, s))) return rc;
-    std::vector<unsigned int> qz(std::max<size_t>(1, ctx->cps.size()), 0u);
+    // two work-queue heads per class pair: the FP64 register kernel's, then the FP32 one's
+    std::vector<unsigned int> qz(2 * std::max<size_t>(1, ctx->cps.size()), 0u);
     if ((rc = dev_upload(&ctx->d_queue, qz, own, s))) return rc;
   }
   HIP_CHECK(hipStreamSynchronize(s));
@@ -1020,6 +1047,7 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   w.lao_offC = ctx->lao_off[cp.lc];
   w.lao_offD = ctx->lao_off[cp.ld];
   w.queue = ctx->d_queue + index;
+  w.queue32 = ctx->d_queue + std::max<size_t>(1, ctx->cps.size()) + index;
   w.wpg = cp.wpg;
   return w;
 }
@@ -1046,15 +1074,21 @@ static hipStream_t stream_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, h
   return (cp.stream > 0 && cp.stream <= (int)ctx->side.size()) ? ctx->side[cp.stream - 1] : s;
 }
 
-static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool timed) {
+// queue: the class pairs' work-queue heads (the context's unless a caller
+// passes others); dev: the device tables the kernels read
+static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool timed, unsigned int* queue = nullptr,
+                       const DevBasis* dev = nullptr) {
+  if (!queue) queue = ctx->d_queue;
+  if (!dev) dev = &ctx->dev;
   if (timed) {
     if (!ctx->ev_start) HIP_CHECK(hipEventCreate(&ctx->ev_start));
     if (!ctx->ev_stop) HIP_CHECK(hipEventCreate(&ctx->ev_stop));
     HIP_CHECK(hipEventRecord(ctx->ev_start, s));
   }
+  const long long ncp = (long long)std::max<size_t>(1, ctx->cps.size());
   {  // work-queue heads start at zero (a kernel, not a memset: the build may be graph-captured)
-    const long long nq = (long long)std::max<size_t>(1, ctx->cps.size());
-    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, ctx->d_queue, nq);
+    const long long nq = 2 * ncp;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, queue, nq);
     HIP_CHECK(hipGetLastError());
   }
   int rc = fork_streams(ctx, s);
@@ -1064,6 +1098,8 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     const ClassPairHost& cph = ctx->cps[i];
     hipStream_t ls = stream_of(ctx, cph, s);
     ClassWork w = work_of(ctx, cph, i);
+    w.queue = queue + i;
+    w.queue32 = queue + ncp + i;
     if (o.D2 && w.acc_colsC > 0) {  // two accumulator sets must fit the workgroup's LDS
       const size_t nA = ncart_c(w.la), nB = ncart_c(w.lb);
       const size_t bytes = 2 * sizeof(double) * (nA * nB + (nA + nB) * (size_t)(w.acc_colsC + w.acc_colsD));
@@ -1071,9 +1107,9 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
         w.acc_colsC = w.acc_colsD = 0;
     }
     bool handled = false;
-    HIP_CHECK(launch_eri_fast(ctx->dev, w, o, ls, &handled));
-    if (!handled) HIP_CHECK(launch_eri_coop(ctx->dev, w, o, ls, &handled));
-    if (!handled) HIP_CHECK(launch_eri_generic(ctx->dev, w, o, ls));
+    HIP_CHECK(launch_eri_fast(*dev, w, o, ls, &handled));
+    if (!handled) HIP_CHECK(launch_eri_coop(*dev, w, o, ls, &handled));
+    if (!handled) HIP_CHECK(launch_eri_generic(*dev, w, o, ls));
     ++n;
   }
   if ((rc = join_streams(ctx, s))) return rc;
@@ -1090,14 +1126,19 @@ static inline unsigned grid_for(long long n, int block) {
 
 // J/K of one density (D2 == nullptr) or of two densities from one evaluation
 // of each quartet; the outputs are zeroed first and symmetrised at the end.
+// queue, dsp (both or neither): per-build scratch in place of the context's
+// work-queue heads and shell-pair density maxima.
 static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, double* J, double* K, double* J2,
-                         double* K2, bool wantJ, bool wantK, int flags, hipStream_t s, bool symmetrize = true) {
+                         double* K2, bool wantJ, bool wantK, int flags, hipStream_t s, bool symmetrize = true,
+                         unsigned int* queue = nullptr, double* dsp = nullptr) {
   const long long nn = ctx->nelem;
   double* outs[4] = {wantJ ? J : nullptr, wantK ? K : nullptr, (D2 && wantJ) ? J2 : nullptr,
                      (D2 && wantK) ? K2 : nullptr};
   for (double* X : outs)
     if (X) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, X, nn, 0.0);
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, D2, ctx->dev, ctx->d_Dsp);
+  DevBasis dev = ctx->dev;
+  if (dsp) dev.Dsp = dsp;
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, D2, dev, (double*)dev.Dsp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_JK;
@@ -1118,7 +1159,7 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
     hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(nc, 256)), dim3(256), 0, s, ctx->d_counters, nc);
     o.counters = ctx->d_counters;
   }
-  int rc = run_classes(ctx, o, s, (flags & ASUKA_JK_TIME) != 0);
+  int rc = run_classes(ctx, o, s, (flags & ASUKA_JK_TIME) != 0, queue, &dev);
   if (rc) return rc;
   if (!symmetrize) return ASUKA_OK;
   const int n = ctx->nao;
@@ -1322,6 +1363,7 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
     for (int r = 0; r <= reps; ++r) {
       ClassWork w = work_of(ctx, ctx->cps[i], i);
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, w.queue, 1LL);
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, w.queue32, 1LL);
       HIP_CHECK(hipEventRecord(e0, s));
       bool handled = false;
       HIP_CHECK(launch_eri_fast(ctx->dev, w, o, s, &handled));

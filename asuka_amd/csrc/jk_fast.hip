@@ -18,7 +18,8 @@ int find_launcher_p4_nch(int la, int lb, int lc, int ld);
 int find_launcher_p5_nch(int la, int lb, int lc, int ld);
 int find_launcher_p6_nch(int la, int lb, int lc, int ld);
 
-// Component chunks of a class's plan (each wave unit is that many wave items); 0 without a plan.
+// Wave items per wave unit of a class's plan (its component chunks, or its C
+// chunks when the plan carries J[c,d] across A chunks); 0 without a plan.
 int eri_fast_nch(int la, int lb, int lc, int ld) {
   int (*finders[])(int, int, int, int) = {find_launcher_p0_nch, find_launcher_p1_nch, find_launcher_p2_nch,
                                           find_launcher_p3_nch, find_launcher_p4_nch, find_launcher_p5_nch,

@@ -284,7 +284,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
       // 2 pi^(5/2) cK_ab cK_cd / (p q sqrt(p+q))
       const CT base = CT(X.ck) * ckq * rsq_nr(spq);
       CT rx[NR], rw[NR];
-      rys_roots_blk<NR, CT, (NR <= kLdsRootsMax ? rys_c::lds_stride(NR, sizeof(CT)) : rys_c::interval_size(NR))>(
+      rys_roots_blk<NR, CT, (NR <= kLdsRootsMax ? rys_c::lds_stride_of<CT>(NR) : rys_c::interval_size_of<CT>(NR))>(
           T, tab, rx, rw);
       const CT qf = q * inv_pq, pf = p * inv_pq;
 #pragma unroll
@@ -615,15 +615,19 @@ __device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const 
 // task's J[a,b] block and K rows of a and b are summed in LDS and added to
 // J/K once at the end of the task.
 //
-// Precision: with o.mixed a wave runs the FP32 instantiation when every active
-// quartet's bound Q_ab Q_cd max|D| is below tau, else the FP64 one; the vote
-// is uniform, so the wave never executes both.
+// Precision: CT is the kernel's arithmetic. Without o.mixed only the FP64
+// kernel runs and takes every unit. With o.mixed both run, each walking the
+// whole task list from its own queue head: a unit goes to the FP32 kernel when
+// every active quartet's bound Q_ab Q_cd max|D| is below tau (a wave-uniform
+// vote), else to the FP64 kernel, and the other kernel skips it. Separate
+// kernels keep the FP32 path's register allocation (and so its occupancy)
+// its own.
 //
 // The read-only inputs are also passed as __restrict__ arguments and rebound
 // into b, w and o: with no possible aliasing store (the J/K atomics) the
 // compiler can turn wave-uniform loads (wave unit, bra pair, bra primitive
 // pairs, D blocks of the bra) into scalar loads.
-template <class PL>
+template <class PL, typename CT>
 __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o,
                                                        const PrimPairF* __restrict__ ppf,
                                                        const WaveUnit* __restrict__ units,
@@ -648,24 +652,26 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
   w.units = units;
   w.col_pair = col_pair;
   o.D = Dm;
+  constexpr bool kF32 = sizeof(CT) == 4;
   constexpr int NIT = PL::CARRY ? PL::SPLIT_C : PL::NCH, NR = PL::NR;  // wave items per unit
   constexpr int NA = nc_(PL::la), NB = PL::NB;
   constexpr bool kLds = NR <= kLdsRootsMax;
-  // LDS copies of the root tables, intervals padded (rys_c::lds_stride)
-  constexpr int NFI = rys_c::interval_size(NR);
-  constexpr int S64 = rys_c::lds_stride(NR, 8), S32 = rys_c::lds_stride(NR, 4);
-  constexpr int TB = kLds ? rys_c::block_size(NR) : 1;
-  constexpr int TB64 = kLds ? rys_c::nint[NR] * S64 : 1, TB32 = kLds ? rys_c::nint[NR] * S32 : 1;
+  // LDS copy of the CT root table, intervals padded (rys_c::lds_stride_of)
+  constexpr int NFI = rys_c::interval_size_of<CT>(NR);
+  constexpr int SCT = rys_c::lds_stride_of<CT>(NR);
+  constexpr int TB = kLds ? rys_c::block_size_of<CT>(NR) : 1;
+  constexpr int TBC = kLds ? rys_c::nint_of<CT>(NR) * SCT : 1;
   const int wpg = w.wpg, nthr = 64 * w.wpg;  // the launch's block size (uniform)
-  __shared__ double t64[TB64];
-  __shared__ float t32[TB32];
-  const double* g64 = rys_dev::rys_coef + rys_c::coef_off[NR];
-  const float* g32 = rys_dev_f::rys_coef + rys_c::coef_off[NR];
+  __shared__ CT tct[TBC];
+  const CT* gct;
+  if constexpr (kF32)
+    gct = rys_dev_f::rys32_coef + rys_c::coef_off32[NR];
+  else
+    gct = rys_dev::rys_coef + rys_c::coef_off[NR];
   if constexpr (kLds) {
     for (int t = threadIdx.x; t < TB; t += nthr) {
       const int iv = t / NFI, f = t - iv * NFI;
-      t64[iv * S64 + f] = g64[t];
-      t32[iv * S32 + f] = g32[t];
+      tct[iv * SCT + f] = gct[t];
     }
   }
   const bool use_acc = w.acc_colsC > 0;
@@ -686,21 +692,23 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
     acc.stride = nacc;
     for (int t = threadIdx.x; t < nacc * nden; t += nthr) s_acc[t] = 0.0;
   }
+  __shared__ int s_task, s_work;
+  if (threadIdx.x == 0) s_work = 0;
   __syncthreads();
-  const double* tab64 = kLds ? t64 : g64;
-  const float* tab32 = kLds ? t32 : g32;
+  const CT* tab = kLds ? tct : gct;
   const int lane = threadIdx.x & 63;
+  unsigned int* queue = kF32 ? w.queue32 : w.queue;
   // launched with kGroupThreads-thread blocks; blockDim is not read because
   // the compiler cannot prove it uniform, which would make the loops divergent
   const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  __shared__ int s_task;
   for (;;) {  // persistent workgroups pull row tasks from the class pair's queue
-    if (threadIdx.x == 0) s_task = (int)atomicAdd(w.queue, 1u);
+    if (threadIdx.x == 0) s_task = (int)atomicAdd(queue, 1u);
     __syncthreads();
     const int t = s_task;
     if (t >= w.ntasks) break;  // uniform per workgroup; every workgroup gets here
     const RowTask tk = w.tasks[t];
     const int P = uni(w.units[tk.u0].P);
+    bool worked = false;  // this wave digested an item of the task (uniform)
     // a wave item is one chunk of a unit, or for CARRY plans one C chunk with
     // its A chunks in sequence; items chunk-major: the waves of a workgroup
     // run the same chunk's code
@@ -719,6 +727,8 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       }
       if (!wave_any(active)) continue;  // the whole unit is density-screened
       const bool lowp = o.mixed && wave_all(!active || bound < o.tau_mixed);
+      if (lowp != kF32) continue;  // the other precision's kernel takes this unit
+      worked = true;
       if (o.counters && chunk == 0) {
         // statistics: one atomic per counter and wave
         const int A = b.sp_A[P], B = b.sp_B[P], C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
@@ -734,7 +744,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
         const unsigned long long nact = (unsigned long long)__popcll(__ballot(active));
         if (lane == 0) {
           atomicAdd(cnt + 8, ksum);
-          if (lowp) {
+          if (kF32) {
             atomicAdd(cnt + 9, nact);
             atomicAdd(cnt + 10, ksum);
           }
@@ -744,54 +754,55 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       // atomics so the wave stays converged for the cross-lane reductions
       // bra primitive pairs this unit evaluates (see run_chunk)
       const int nab = uni(o.eps_prim > 0.0 ? un.nab : b.pp_count[P]);
-      if (lowp)
-        dispatch_citem<PL, 0, float>(chunk, b, o, P, Qp, nab, active, tab32, acc);
-      else
-        dispatch_citem<PL, 0, double>(chunk, b, o, P, Qp, nab, active, tab64, acc);
+      dispatch_citem<PL, 0, CT>(chunk, b, o, P, Qp, nab, active, tab, acc);
     }
     if (use_acc) {
+      if (worked && lane == 0) s_work = 1;
       __syncthreads();  // the task's contributions are in LDS
-      const int A = b.sp_A[P], B = b.sp_B[P];
-      long long N, msh;
-      mol_layout(b, A, N, msh);
-      const long long aoA = b.shell_ao[A], aoB = b.shell_ao[B];
-      const int cC = w.acc_colsC, cD = w.acc_colsD;
-      const int* mlao = b.mol_lao + b.shell_mol[A] * (ASUKA_LMAX + 1);  // the molecule's AO lists
-      const int* laoC = b.lao_ao + w.lao_offC + mlao[PL::lc];
-      const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD + mlao[PL::ld] : w.lao_offC + mlao[PL::lc]);
-      const int cDe = cD > 0 ? cD : cC;
-      for (int t2 = threadIdx.x; t2 < nacc * nden; t2 += nthr) {
-        const double v = s_acc[t2];
-        if (v == 0.0) continue;
-        s_acc[t2] = 0.0;
-        const bool second = t2 >= nacc;
-        double* Jo = shifted(second ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
-        double* Ko = shifted(second ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
-        int r = second ? t2 - nacc : t2;
-        if (r < NA * NB) {
-          atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], v);
-          continue;
+      if (s_work) {
+        const int A = b.sp_A[P], B = b.sp_B[P];
+        long long N, msh;
+        mol_layout(b, A, N, msh);
+        const long long aoA = b.shell_ao[A], aoB = b.shell_ao[B];
+        const int cC = w.acc_colsC, cD = w.acc_colsD;
+        const int* mlao = b.mol_lao + b.shell_mol[A] * (ASUKA_LMAX + 1);  // the molecule's AO lists
+        const int* laoC = b.lao_ao + w.lao_offC + mlao[PL::lc];
+        const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD + mlao[PL::ld] : w.lao_offC + mlao[PL::lc]);
+        const int cDe = cD > 0 ? cD : cC;
+        for (int t2 = threadIdx.x; t2 < nacc * nden; t2 += nthr) {
+          const double v = s_acc[t2];
+          if (v == 0.0) continue;
+          s_acc[t2] = 0.0;
+          const bool second = t2 >= nacc;
+          double* Jo = shifted(second ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
+          double* Ko = shifted(second ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
+          int r = second ? t2 - nacc : t2;
+          if (r < NA * NB) {
+            atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], v);
+            continue;
+          }
+          r -= NA * NB;
+          if (r < NA * cC) {
+            atomicAdd(&Ko[(aoA + r / cC) * N + laoC[r % cC]], v);
+            continue;
+          }
+          r -= NA * cC;
+          if (cD > 0 && r < NA * cD) {
+            atomicAdd(&Ko[(aoA + r / cD) * N + laoD[r % cD]], v);
+            continue;
+          }
+          if (cD > 0) r -= NA * cD;
+          if (r < NB * cC) {
+            atomicAdd(&Ko[(aoB + r / cC) * N + laoC[r % cC]], v);
+            continue;
+          }
+          r -= NB * cC;
+          atomicAdd(&Ko[(aoB + r / cDe) * N + laoD[r % cDe]], v);
         }
-        r -= NA * NB;
-        if (r < NA * cC) {
-          atomicAdd(&Ko[(aoA + r / cC) * N + laoC[r % cC]], v);
-          continue;
-        }
-        r -= NA * cC;
-        if (cD > 0 && r < NA * cD) {
-          atomicAdd(&Ko[(aoA + r / cD) * N + laoD[r % cD]], v);
-          continue;
-        }
-        if (cD > 0) r -= NA * cD;
-        if (r < NB * cC) {
-          atomicAdd(&Ko[(aoB + r / cC) * N + laoC[r % cC]], v);
-          continue;
-        }
-        r -= NB * cC;
-        atomicAdd(&Ko[(aoB + r / cDe) * N + laoD[r % cDe]], v);
       }
     }
-    __syncthreads();  // accumulators are zero again; s_task may be overwritten
+    __syncthreads();  // accumulators are zero again; s_task and s_work may be overwritten
+    if (use_acc && threadIdx.x == 0) s_work = 0;
   }
 }
 
@@ -802,6 +813,36 @@ inline size_t row_acc_bytes(int la, int lb, int colsC, int colsD) {
   return sizeof(double) * (size_t)(NA * NB + (NA + NB) * (colsC + colsD));
 }
 
+// The FP32 kernel's plan of a class: the FP64 plan's component chunks, or
+// (built with ASUKA_F32_HALVE) half as many, the FP32 intermediates taking
+// half the registers: A chunks halved when their count is even, else C chunks.
+#ifndef ASUKA_F32_HALVE
+#define ASUKA_F32_HALVE 0
+#endif
+template <class PL>
+struct F32PlanOf {
+  static constexpr bool halve_a = ASUKA_F32_HALVE && PL::SPLIT_A % 2 == 0;
+  static constexpr bool halve_c = ASUKA_F32_HALVE && !halve_a && PL::SPLIT_C % 2 == 0;
+  static constexpr int SA = halve_a ? PL::SPLIT_A / 2 : PL::SPLIT_A;
+  static constexpr int SC = halve_c ? PL::SPLIT_C / 2 : PL::SPLIT_C;
+  using type = Plan<PL::la, PL::lb, PL::lc, PL::ld, SA, SC>;
+};
+
+template <class PL, typename CT>
+hipError_t launch_one(const DevBasis& b, const ClassWork& w, const EriOut& o, size_t lds, long long g, hipStream_t s) {
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void*)eri_fast_kernel<PL, CT>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((eri_fast_kernel<PL, CT>), dim3((unsigned)g), dim3(64 * w.wpg), lds, s, b, w, o, b.ppf,
+                     w.units, w.col_pair, b.sp_A, b.sp_B, b.pp_start, b.pp_count, b.shell_xyz, b.shell_ao, b.Q,
+                     b.Dsp, o.D);
+  return hipGetLastError();
+}
+
+// The FP64 kernel of a class pair and, for mixed builds, the FP32 kernel after
+// it on the same stream (each takes its own units, see eri_fast_kernel).
 template <class PL>
 hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, hipStream_t s) {
   long long g = w.ntasks;  // persistent: a few workgroups per CU, tasks from the queue
@@ -810,14 +851,9 @@ hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, h
   if (g < 1) g = 1;
   const size_t lds =
       w.acc_colsC > 0 ? (o.D2 ? 2 : 1) * row_acc_bytes(PL::la, PL::lb, w.acc_colsC, w.acc_colsD) : 0;
-  if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute((const void*)eri_fast_kernel<PL>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL((eri_fast_kernel<PL>), dim3((unsigned)g), dim3(64 * w.wpg), lds, s, b, w, o, b.ppf, w.units,
-                     w.col_pair, b.sp_A, b.sp_B, b.pp_start, b.pp_count, b.shell_xyz, b.shell_ao, b.Q, b.Dsp, o.D);
-  return hipGetLastError();
+  hipError_t e = launch_one<PL, double>(b, w, o, lds, g, s);
+  if (e != hipSuccess || !o.mixed) return e;
+  return launch_one<typename F32PlanOf<PL>::type, float>(b, w, o, lds, g, s);
 }
 
 using Launcher = hipError_t (*)(const DevBasis&, const ClassWork&, const EriOut&, hipStream_t);
@@ -825,7 +861,9 @@ using Launcher = hipError_t (*)(const DevBasis&, const ClassWork&, const EriOut&
 }  // namespace asuka_fast
 
 // Each jk_fast_*.hip translation unit instantiates a subset of class plans;
-// FN returns a plan's launcher, FN##_nch its number of component chunks.
+// FN returns a plan's launcher, FN##_nch its wave items per unit (the number
+// of component chunks, or of C chunks for plans that carry J[c,d] across
+// their A chunks).
 #define ASUKA_FAST_FIND(FN, LIST)                                                     \
   asuka_fast::Launcher FN(int la, int lb, int lc, int ld) {                          \
     using namespace asuka_fast;                                                       \
@@ -839,4 +877,4 @@ using Launcher = hipError_t (*)(const DevBasis&, const ClassWork&, const EriOut&
 #define ASUKA_FAST_CASE(a, b_, c, d, sa, sc) \
   if (la == a && lb == b_ && lc == c && ld == d) return &launch_plan<Plan<a, b_, c, d, sa, sc>>;
 #define ASUKA_FAST_NCH_CASE(a, b_, c, d, sa, sc) \
-  if (la == a && lb == b_ && lc == c && ld == d) return (sa) * (sc);
+  if (la == a && lb == b_ && lc == c && ld == d) return asuka_fast::Plan<a, b_, c, d, sa, sc>::CARRY ? (sc) : (sa) * (sc);

@@ -235,6 +235,18 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
           o.tile[w.q_out_off[k] + comp] = v;
           continue;
         }
+        if (o.mode == MODE_X3C) {  // ket = (aux shell C, unit s): icd is the aux component
+          const int ia = iab / nB, ib = iab - ia * nB;
+          const int a = b.shell_ao[A] + ia, bb = b.shell_ao[B] + ib;
+          double* Xp = o.X + (long long)(b.shell_ao[C] + icd) * o.x_nao * o.x_nao;
+          Xp[(long long)a * o.x_nao + bb] = v;
+          if (A != B) Xp[(long long)bb * o.x_nao + a] = v;
+          continue;
+        }
+        if (o.mode == MODE_V2C) {  // bra = (aux shell A, unit s), ket = (aux shell C, unit s)
+          o.X[(long long)(b.shell_ao[A] + iab) * o.x_naux + b.shell_ao[C] + icd] = v;
+          continue;
+        }
         if (o.mode == MODE_SCHWARZ) {
           if (iab == icd && v > qmax) {
             qmax = v;

@@ -42,6 +42,8 @@ enum EriMode : int {
   MODE_FOCK = 1,     // accumulate J - K/2 into F
   MODE_TILE = 2,     // write the tile
   MODE_SCHWARZ = 3,  // Q[P] = sqrt(max diag (P|P))
+  MODE_X3C = 4,      // three-centre (ab|P): X[P][a][b] (and [b][a]), ket pair = (aux shell, unit s)
+  MODE_V2C = 5,      // two-centre (P|Q): V[P][Q], bra and ket pairs = (aux shell, unit s)
 };
 
 // Device-side view of a context (passed by value to kernels).
@@ -112,6 +114,7 @@ struct ClassWork {
   // where their AO lists start in lao_ao; acc_colsC == 0 disables them.
   int acc_colsC, acc_colsD, lao_offC, lao_offD;
   unsigned int* queue;  // work-queue head (row tasks handed out in order), zero at launch
+  unsigned int* queue32;  // the FP32 register kernel's own head (mixed builds), zero at launch
   int wpg;              // wavefronts per workgroup of the register kernels (4 or 8)
   const Segment* seg;
   int nseg;
@@ -123,6 +126,10 @@ struct ClassWork {
   const int* q_ab;
   const int* q_cd;
   const long long* q_out_off;
+  // product list (three- and two-centre integrals): work index k is the
+  // quartet (q_bra0 + k / q_nket, q_ket0 + k % q_nket) when q_nket > 0
+  long long q_nket;
+  int q_bra0, q_ket0;
   int cp_index;  // class-pair index (statistics counters)
 };
 
@@ -147,6 +154,10 @@ struct EriOut {
   const double* D2;
   double* J2;
   double* K2;
+  // MODE_X3C: X (x_naux, x_nao, x_nao); MODE_V2C: X (x_naux, x_naux); AO and
+  // auxiliary function indices are the shells' shell_ao
+  double* X;
+  long long x_nao, x_naux;
 };
 
 constexpr int kCountStride = 16;
@@ -171,6 +182,11 @@ __host__ __device__ constexpr inline int ncart_c(int l) { return (l + 1) * (l + 
 // Decode work index k of a class pair into (bra pair, ket pair); false for
 // segment padding.
 __device__ inline bool decode_work(const ClassWork& w, long long k, int& P, int& Qp) {
+  if (w.q_nket > 0) {
+    P = w.q_bra0 + (int)(k / w.q_nket);
+    Qp = w.q_ket0 + (int)(k % w.q_nket);
+    return true;
+  }
   if (w.q_ab) {
     P = w.q_ab[k];
     Qp = w.q_cd[k];

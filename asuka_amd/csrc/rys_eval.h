@@ -17,12 +17,11 @@ namespace rys_dev {
 #undef RYS_TABLE_QUAL
 }  // namespace rys_dev
 
-// FP32 copy of the polynomial coefficients (same layout) for the FP32 path
+// The FP32 path's own table (gen_rys_table_f32.py): degree-RYS32_DEG
+// polynomials on the same intervals up to its own, lower asymptotic switch.
 namespace rys_dev_f {
 #define RYS_TABLE_QUAL static const __device__
-#define RYS_COEF_T float
-#include "rys_table.h"
-#undef RYS_COEF_T
+#include "rys_table_f32.h"
 #undef RYS_TABLE_QUAL
 }  // namespace rys_dev_f
 
@@ -92,39 +91,6 @@ __device__ inline void rys_roots_dev(double T, double* x, double* w) {
   }
 }
 
-// All n roots in FP32 (the FP32 path of mixed precision). Below the
-// asymptotic switch the FP32 coefficient copy is evaluated in FP32; at large T
-// (where the Gauss-Hermite limit applies) x and w are formed in FP64 and
-// rounded, so the large-argument roots keep FP64 accuracy before rounding.
-template <int N>
-__device__ inline void rys_roots_dev_f(float T, float* x, float* w) {
-  if (T >= (float)rys_dev::rys_tmax[N]) {
-    const double it = 1.0 / (double)T;
-    const double sit = sqrt(it);
-#pragma unroll
-    for (int u = 0; u < N; ++u) {
-      x[u] = (float)(rys_dev::rys_herm_x[rys_dev::rys_herm_off[N] + u] * it);
-      w[u] = (float)(rys_dev::rys_herm_w[rys_dev::rys_herm_off[N] + u] * sit);
-    }
-    return;
-  }
-  int i = (int)(T * (float)RYS_INV_H);
-  if (i >= rys_dev::rys_nint[N]) i = rys_dev::rys_nint[N] - 1;
-  const float s = (T - (float)i * (float)RYS_H) * (float)(2.0 * RYS_INV_H) - 1.0f;
-  const float* c = rys_dev_f::rys_coef + rys_dev::rys_coef_off[N] + i * 2 * N * (RYS_DEG + 1);
-#pragma unroll
-  for (int f = 0; f < 2 * N; ++f) {
-    const float* cf = c + f * (RYS_DEG + 1);
-    float a = cf[RYS_DEG];
-#pragma unroll
-    for (int d = RYS_DEG - 1; d >= 0; --d) a = __builtin_fmaf(a, s, cf[d]);
-    if (f < N)
-      x[f] = a;
-    else
-      w[f - N] = a;
-  }
-}
-
 // Compile-time copies of the table metadata.
 namespace rys_c {
 constexpr int nint[] = {RYS_NINT_LIST};
@@ -144,9 +110,34 @@ constexpr int lds_stride(int n, int esize) {
   if ((s / per16) % 2 == 0) s += per16;
   return s;
 }
-// LDS bytes of the padded FP64 and FP32 copies of the n-root block
-constexpr size_t lds_table_bytes(int n) {
-  return (size_t)nint[n] * ((size_t)lds_stride(n, 8) * 8 + (size_t)lds_stride(n, 4) * 4);
+// LDS bytes of the padded FP64 copy of the n-root block (the larger of a
+// kernel's two possible copies)
+constexpr size_t lds_table_bytes(int n) { return (size_t)nint[n] * (size_t)lds_stride(n, 8) * 8; }
+
+// The table a kernel of arithmetic CT reads: the FP64 one, or for float the
+// FP32 table (degree RYS32_DEG, its own asymptotic switch and interval count).
+constexpr int nint32[] = {RYS32_NINT_LIST};
+constexpr double tmax32[] = {RYS32_TMAX_LIST};
+constexpr int coef_off32[] = {RYS32_COEF_OFF_LIST};
+template <typename CT>
+constexpr int deg_of() { return sizeof(CT) == 4 ? RYS32_DEG : RYS_DEG; }
+template <typename CT>
+constexpr int nint_of(int n) { return sizeof(CT) == 4 ? nint32[n] : nint[n]; }
+template <typename CT>
+constexpr double tmax_of(int n) { return sizeof(CT) == 4 ? tmax32[n] : tmax[n]; }
+template <typename CT>
+constexpr int coef_off_of(int n) { return sizeof(CT) == 4 ? coef_off32[n] : coef_off[n]; }
+template <typename CT>
+constexpr int interval_size_of(int n) { return 2 * n * (deg_of<CT>() + 1); }
+template <typename CT>
+constexpr int block_size_of(int n) { return nint_of<CT>(n) * interval_size_of<CT>(n); }
+// interval stride of a padded LDS copy, as lds_stride
+template <typename CT>
+constexpr int lds_stride_of(int n) {
+  constexpr int per16 = 16 / (int)sizeof(CT);
+  int s = (interval_size_of<CT>(n) + per16 - 1) / per16 * per16;
+  if ((s / per16) % 2 == 0) s += per16;
+  return s;
 }
 static_assert(RYS_H == 2.0, "the block evaluators assume intervals of width 2");
 }  // namespace rys_c
@@ -174,15 +165,17 @@ __device__ __forceinline__ double rsq_nr(double x) {
 __device__ __forceinline__ float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq_nr(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// All N roots and weights at T from the N-root coefficient block `blk`
-// (rys_coef + coef_off[N], or a copy of it in LDS), in the arithmetic of CT.
-// At T >= tmax[N] the Gauss-Hermite limit, with 1/T and 1/sqrt(T) in CT.
-// STRIDE is the interval stride of blk: interval_size(N) for the global
-// table, lds_stride(N, sizeof(CT)) for a padded LDS copy.
-template <int N, typename CT, int STRIDE = 2 * N * (RYS_DEG + 1)>
+// All N roots and weights at T from the N-root coefficient block `blk` of
+// the CT table (rys_coef + coef_off[N] for double, rys32_coef + coef_off32[N]
+// for float, or a copy of it in LDS), in the arithmetic of CT. At T >=
+// tmax_of<CT>(N) the Gauss-Hermite limit, with 1/T and 1/sqrt(T) in CT.
+// STRIDE is the interval stride of blk: interval_size_of<CT>(N) for the global
+// table, lds_stride_of<CT>(N) for a padded LDS copy.
+template <int N, typename CT, int STRIDE = rys_c::interval_size_of<CT>(N)>
 __device__ __forceinline__ void rys_roots_blk(CT T, const CT* __restrict__ blk, CT* x, CT* w) {
-  constexpr int NI = rys_c::nint[N];
-  if (T >= CT(rys_c::tmax[N])) {
+  constexpr int NI = rys_c::nint_of<CT>(N);
+  constexpr int DEG = rys_c::deg_of<CT>();
+  if (T >= CT(rys_c::tmax_of<CT>(N))) {
     const CT it = rcp_nr(T);
     const CT sit = rsq_nr(T);
 #pragma unroll
@@ -198,10 +191,10 @@ __device__ __forceinline__ void rys_roots_blk(CT T, const CT* __restrict__ blk, 
   const CT* c = blk + i * STRIDE;
 #pragma unroll
   for (int f = 0; f < 2 * N; ++f) {
-    const CT* cf = c + f * (RYS_DEG + 1);
-    CT a = cf[RYS_DEG];
+    const CT* cf = c + f * (DEG + 1);
+    CT a = cf[DEG];
 #pragma unroll
-    for (int d = RYS_DEG - 1; d >= 0; --d) a = fma(a, s, cf[d]);
+    for (int d = DEG - 1; d >= 0; --d) a = fma(a, s, cf[d]);
     if (f < N)
       x[f] = a;
     else

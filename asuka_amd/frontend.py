@@ -77,7 +77,8 @@ def run_rhf_direct(mol, *, basis: Any | None = None, eps_schwarz: float = 1e-12,
                    conv_tol_dm: float = 1e-8, diis: bool = True, diis_start_cycle: int | None = None,
                    diis_space: int = 8, damping: float = 0.0, level_shift: float = 0.0, dm0=None, mo_coeff0=None,
                    init_fock_cycles: int | None = None, mixed_precision: bool | None = None,
-                   profile: dict | None = None) -> RHFDFRunResult:
+                   profile: dict | None = None, use_graph: bool | None = None,
+                   check_every: int = 4) -> RHFDFRunResult:
     if int(getattr(mol, "spin", 0)) != 0:
         raise NotImplementedError("run_rhf_direct currently supports only closed-shell molecules (spin=0)")
     nelec = int(mol.nelectron)
@@ -99,10 +100,55 @@ def run_rhf_direct(mol, *, basis: Any | None = None, eps_schwarz: float = 1e-12,
                      max_cycle=int(max_cycle), conv_tol=float(conv_tol), conv_tol_dm=float(conv_tol_dm),
                      diis=bool(diis), diis_start_cycle=dstart, diis_space=int(diis_space), damping=float(damping),
                      level_shift=float(level_shift), dm0=dm0, mo_coeff0=mo_coeff0, init_fock_cycles=nin,
-                     mixed_precision=mixed_precision, profile=profile)
+                     mixed_precision=mixed_precision, profile=profile, use_graph=use_graph,
+                     check_every=int(check_every))
     return RHFDFRunResult(mol=mol, basis_name=str(basis_name), auxbasis_name="<direct>", ao_basis=ao_basis,
                           aux_basis=None, int1e=int1e, df_B=None, scf=scf, profile=profile, two_e_backend="direct",
                           direct_jk_ctx=jk_ctx)
+
+
+def run_rhf_df(mol, *, basis: Any | None = None, auxbasis: Any | None = None, max_cycle: int = 50,
+               conv_tol: float = 1e-10, conv_tol_dm: float = 1e-8, diis: bool = True,
+               diis_start_cycle: int | None = None, diis_space: int = 8, damping: float = 0.0,
+               level_shift: float = 0.0, dm0=None, mo_coeff0=None, init_fock_cycles: int | None = None,
+               profile: dict | None = None, **_ignored) -> RHFDFRunResult:
+    """RHF with density-fitted J/K (the reference's default backend,
+    asuka/frontend/_scf_dispatch.py:68-71 -> asuka/hf/df_scf.py:871 rhf_df):
+    auxiliary metric and three-centre integrals on the GPU (asuka_amd.df),
+    whitened B = L^-1 X, and J/K from B by dense matrix products each cycle.
+    ``auxbasis``: an explicit basis dict, or None / "etb" for the even-tempered
+    set generated from the orbital basis."""
+    from .df import build_aux_basis_cart, df_JK, df_tensors
+
+    if int(getattr(mol, "spin", 0)) != 0:
+        raise NotImplementedError("run_rhf_df supports only closed-shell molecules (spin=0)")
+    nelec = int(mol.nelectron)
+    if nelec <= 0 or nelec % 2 != 0:
+        raise ValueError("RHF requires an even, positive electron count")
+    ao_basis, basis_name = build_ao_basis_cart(mol, basis)
+    aux_basis = build_aux_basis_cart(mol, auxbasis, basis if basis is not None else mol.basis)
+    jk_ctx = make_direct_jk_context(ao_basis, eps_schwarz=1e-12, atom_coords_bohr=mol.coords_bohr,
+                                    atom_charges=mol.charges)
+    int1e = build_int1e(jk_ctx)
+    _, _, _, BQ = df_tensors(ao_basis, aux_basis)
+    h = int1e.hcore
+
+    def fock(D):
+        J, K = df_JK(BQ, D)
+        return h + J - 0.5 * K
+
+    default_init = int(os.environ.get("ASUKA_HF_INIT_FOCK_CYCLES", "1") or 1)
+    nin = default_init if init_fock_cycles is None else max(0, int(init_fock_cycles))
+    dstart = int(diis_start_cycle) if diis_start_cycle is not None else (1 if nin > 0 else 2)
+    scf = rhf_direct(int1e.S, h, None, nelec=nelec, enuc=float(mol.energy_nuc()), max_cycle=int(max_cycle),
+                     conv_tol=float(conv_tol), conv_tol_dm=float(conv_tol_dm), diis=bool(diis),
+                     diis_start_cycle=dstart, diis_space=int(diis_space), damping=float(damping),
+                     level_shift=float(level_shift), dm0=dm0, mo_coeff0=mo_coeff0, init_fock_cycles=nin,
+                     profile=profile, use_graph=False, fock_fn=fock)
+    aux_name = auxbasis if isinstance(auxbasis, str) else ("etb" if auxbasis is None else "<explicit>")
+    return RHFDFRunResult(mol=mol, basis_name=str(basis_name), auxbasis_name=str(aux_name), ao_basis=ao_basis,
+                          aux_basis=aux_basis, int1e=int1e, df_B=BQ, scf=scf, profile=profile, two_e_backend="df",
+                          direct_jk_ctx=None)
 
 
 def _run_open_shell_direct(kind: str, mol, *, basis: Any | None = None, eps_schwarz: float = 1e-12,
@@ -161,14 +207,20 @@ def run_hf_df(mol, *, method: str = "rhf", backend: str = "cuda", df: bool = Tru
     backend_s = str(backend).strip().lower()
     if backend_s not in ("cuda", "hip", "gpu"):
         raise ValueError("two_e_backend='direct' requires backend='cuda' (the GPU)")
-    teb = "df" if two_e_backend is None and df else (two_e_backend or "dense")
-    if str(teb).strip().lower() != "direct":
-        raise NotImplementedError(f"two_e_backend={teb!r} is not part of this engine (only 'direct')")
+    teb = str("df" if two_e_backend is None and df else (two_e_backend or "dense")).strip().lower()
+    if teb not in ("direct", "df"):
+        raise NotImplementedError(f"two_e_backend={teb!r} is not part of this engine ('direct' or 'df')")
     if guess is not None:
         if dm0 is None and hasattr(guess, "dm"):
             dm0 = guess.dm
         elif dm0 is None and isinstance(guess, np.ndarray):
             dm0 = guess
+    if teb == "df":
+        if method_s != "rhf":
+            raise NotImplementedError("the density-fitted backend of this engine implements method='rhf'")
+        for k in ("df_layout", "df_threads"):
+            kwargs.pop(k, None)
+        return run_rhf_df(mol, dm0=dm0, mo_coeff0=mo_coeff0, **kwargs)
     for k in ("auxbasis", "df_layout", "df_threads"):
         kwargs.pop(k, None)
     if method_s == "uhf":
@@ -179,4 +231,4 @@ def run_hf_df(mol, *, method: str = "rhf", backend: str = "cuda", df: bool = Tru
 
 
 __all__ = ["Int1eResult", "RHFDFRunResult", "ROHFDFRunResult", "UHFDFRunResult", "build_int1e", "run_hf_df",
-           "run_rhf_direct", "run_rohf_direct", "run_uhf_direct"]
+           "run_rhf_df", "run_rhf_direct", "run_rohf_direct", "run_uhf_direct"]

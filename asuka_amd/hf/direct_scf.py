@@ -30,6 +30,9 @@ import numpy as np
 from .direct_jk import DirectJKContext, direct_fock_rhf, direct_JK, direct_JK_multi
 
 _CPU_EIGH_MAX_NAO = int(os.environ.get("ASUKA_HF_CPU_EIGH_MAX_NAO", "96") or 96)
+# False runs the captured SCF cycle eagerly instead of replaying its graph
+# (the same operations; tests compare the two)
+_GRAPH_REPLAY = True
 
 
 @dataclass(frozen=True)
@@ -160,17 +163,29 @@ def rhf_direct(
     init_fock_cycles: int = 1,
     mixed_precision: bool | None = None,
     profile: dict | None = None,
+    use_graph: bool | None = None,
+    check_every: int = 4,
+    fock_fn=None,
 ) -> SCFResult:
-    """RHF with integral-direct J/K (asuka/hf/direct_scf.py:230-402)."""
+    """RHF with integral-direct J/K (asuka/hf/direct_scf.py:230-402).
+
+    ``use_graph`` (default ``$ASUKA_SCF_GRAPH``, else on when the Fock build
+    serves the basis, max l <= 2): every SCF iteration after the predictor
+    cycles is one replay of a captured HIP graph (Fock build, DIIS with the
+    device solve, density by purification, energy and convergence test), and
+    the host reads the convergence record every ``check_every`` iterations
+    (:func:`_rhf_graph_iterations`)."""
     import torch
 
-    dev = jk_ctx.device
+    dev = jk_ctx.device if jk_ctx is not None else torch.device("cuda", torch.cuda.current_device())
     S = torch.as_tensor(np.asarray(S) if not isinstance(S, torch.Tensor) else S).to(dev, torch.float64)
     h = torch.as_tensor(np.asarray(hcore) if not isinstance(hcore, torch.Tensor) else hcore).to(dev, torch.float64)
     nao = int(S.shape[0])
     if tuple(S.shape) != (nao, nao) or tuple(h.shape) != (nao, nao):
         raise ValueError("S/hcore must be (nao, nao)")
-    if nao != jk_ctx.nao:
+    if jk_ctx is None and fock_fn is None:
+        raise ValueError("rhf_direct needs a DirectJKContext or a fock_fn")
+    if jk_ctx is not None and nao != jk_ctx.nao:
         raise ValueError(f"nao mismatch: S has {nao}, DirectJKContext has {jk_ctx.nao}")
     nelec = int(nelec)
     if nelec < 0 or nelec % 2:
@@ -180,9 +195,11 @@ def rhf_direct(
     occ[:nocc] = 2.0
     X = _orthogonalizer(torch, S)
 
-    use_fock = int(jk_ctx.max_l) <= 2
+    use_fock = fock_fn is None and int(jk_ctx.max_l) <= 2
 
     def fock(D):
+        if fock_fn is not None:  # another two-electron backend (density fitting)
+            return _sym(fock_fn(D))
         if use_fock:
             return direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed_precision)
         J, K = direct_JK(jk_ctx, D, mixed_precision=mixed_precision)
@@ -194,34 +211,58 @@ def rhf_direct(
             return X @ Fp @ X.T
         return F
 
-    eps, C = _gen_eigh(torch, h, X)
+    if use_graph is None:
+        env = os.environ.get("ASUKA_SCF_GRAPH", "").strip()
+        use_graph = (env != "0") if env else use_fock
+    use_graph = bool(use_graph and use_fock and int(diis_start_cycle) <= 1 and int(max_cycle) > 0)
+
+    def guess_density(F):
+        """Aufbau density of F: eigh, or with the graph driver purification
+        (the same projector, no eigensolver)."""
+        if use_graph:
+            Fo = X.T @ shifted(F) @ X
+            P, _ = _purify(torch, _sym(Fo), nocc, _purify_steps(torch, _sym(Fo), nocc) + 4)
+            return None, None, _sym(2.0 * (X @ P @ X.T))
+        e, Cm = _gen_eigh(torch, shifted(F), X)
+        return e, Cm, _density(Cm, nocc)
+
     if dm0 is not None:
         D = _sym(torch.as_tensor(np.asarray(dm0) if not isinstance(dm0, torch.Tensor) else dm0).to(dev, torch.float64))
         if tuple(D.shape) != (nao, nao):
             raise ValueError("dm0 must have shape (nao, nao)")
+        eps = C = None
     elif mo_coeff0 is not None:
         C = torch.as_tensor(np.asarray(mo_coeff0) if not isinstance(mo_coeff0, torch.Tensor) else mo_coeff0).to(
             dev, torch.float64)
         if tuple(C.shape) != (nao, nao):
             raise ValueError("mo_coeff0 must have shape (nao, nao)")
         D = _density(C, nocc)
+        eps = None
     else:
-        D = _density(C, nocc)
+        eps, C, D = guess_density(h)
 
     lam = float(damping) if damping else 0.0
     if init_fock_cycles > 0 and dm0 is None and mo_coeff0 is None:
         for _ in range(int(init_fock_cycles)):
             Dp = D
-            eps, C = _gen_eigh(torch, shifted(fock(Dp)), X)
-            D = _density(C, nocc)
+            eps, C, D = guess_density(fock(Dp))
             if lam:
                 D = (1.0 - lam) * D + lam * Dp
+
+    if use_graph:
+        return _rhf_graph_iterations(torch, S, h, X, D, jk_ctx, nocc=nocc, occ=occ, enuc=float(enuc),
+                                     max_cycle=int(max_cycle), conv_tol=float(conv_tol),
+                                     conv_tol_dm=float(conv_tol_dm), diis=bool(diis), diis_space=int(diis_space),
+                                     damping=lam, level_shift=float(level_shift or 0.0),
+                                     mixed_precision=mixed_precision, check_every=max(1, int(check_every)),
+                                     profile=profile)
 
     diis_obj = _DIIS(torch, diis_space) if diis else None
     e_last = None
     converged = False
     cycle = 0
     njk = 0
+    energies, dm_errors = [], []
     for cycle in range(1, int(max_cycle) + 1):
         F = shifted(fock(D))
         njk += 1
@@ -244,16 +285,214 @@ def rhf_direct(
         dm_err = float(torch.linalg.norm(Dn - D).item())
         de = abs(e_tot - e_last) if e_last is not None else float("inf")
         D, e_last = Dn, e_tot
+        energies.append(e_tot)
+        dm_errors.append(dm_err)
         if de < conv_tol and dm_err < conv_tol_dm:
             converged = True
             break
     if profile is not None:
         profile.setdefault("scf", {})["iters"] = int(cycle)
         profile["scf"]["jk_calls"] = njk
+        profile["scf"]["graph"] = False
+        profile["scf"]["energies"] = energies
+        profile["scf"]["dm_errors"] = dm_errors
     return SCFResult(method="RHF", converged=converged, niter=int(cycle),
                      e_tot=float(e_last if e_last is not None else float("nan")),
                      e_elec=float(e_last - float(enuc)) if e_last is not None else float("nan"),
                      e_nuc=float(enuc), mo_energy=eps, mo_coeff=C, mo_occ=occ, dm=D)
+
+
+_PURIFY_TOL = 1e-11  # ||P^2 - P||_F at which a purification stops updating
+
+
+def _purify(torch, Fo, nocc: int, iters: int):
+    """Projector onto the eigenvectors of the nocc lowest eigenvalues of the
+    symmetric Fo, by trace-correcting purification (TC2; A. M. N. Niklasson,
+    Phys. Rev. B 66, 155115 (2002)): the Gershgorin interval [e0, e1] of Fo
+    maps its spectrum into [0, 1] reversed, P = (e1 - Fo)/(e1 - e0); then each
+    step takes P^2 when Tr P exceeds nocc and 2P - P^2 otherwise, which drives
+    the nocc largest eigenvalues of P (the nocc lowest of Fo) to 1 and the rest
+    to 0 for a nonzero HOMO-LUMO gap. Each map repels one end of [0, 1] (P^2
+    doubles the distance of eigenvalues below 1), so steps past convergence
+    would undo it: once ||P^2 - P||_F falls below _PURIFY_TOL the projector is
+    frozen (a sticky device flag). Matrix products only, and the branch and the
+    freeze are device-side selects, so a fixed number of steps can be captured
+    in a HIP graph. Returns P and the idempotency error of the returned P."""
+    n = int(Fo.shape[0])
+    d = torch.diagonal(Fo)
+    r = Fo.abs().sum(dim=1) - d.abs()
+    e0 = (d - r).min()
+    e1 = (d + r).max()
+    eye = torch.eye(n, dtype=Fo.dtype, device=Fo.device)
+    P = (e1 * eye - Fo) / (e1 - e0)
+    done = torch.zeros((), dtype=torch.bool, device=Fo.device)
+    err = None
+    for _ in range(int(iters)):
+        P2 = P @ P
+        err = torch.linalg.norm(P2 - P)
+        done = done | (err < _PURIFY_TOL)
+        step = torch.where(torch.trace(P) > float(nocc), P2, 2.0 * P - P2)
+        P = torch.where(done, P, step)
+    return _sym(P), err
+
+
+def _purify_steps(torch, Fo, nocc: int, max_steps: int = 300) -> int:
+    """Steps _purify takes on Fo until ||P^2 - P||_F < _PURIFY_TOL (eager, the
+    host reads the error every step)."""
+    n = int(Fo.shape[0])
+    d = torch.diagonal(Fo)
+    r = Fo.abs().sum(dim=1) - d.abs()
+    e0 = (d - r).min()
+    e1 = (d + r).max()
+    P = (e1 * torch.eye(n, dtype=Fo.dtype, device=Fo.device) - Fo) / (e1 - e0)
+    for k in range(1, max_steps + 1):
+        P2 = P @ P
+        if float(torch.linalg.norm(P2 - P)) < _PURIFY_TOL:
+            return k
+        P = P2 if float(torch.trace(P)) > nocc else 2.0 * P - P2
+    raise RuntimeError("density purification did not converge (no HOMO-LUMO gap?)")
+
+
+def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
+                          diis_space, damping, level_shift, mixed_precision, check_every, profile):
+    """The SCF cycles of rhf_direct as replays of one captured HIP graph.
+
+    One replay is one cycle of asuka/hf/direct_scf.py:337-378 with its state
+    in fixed device buffers: the Fock build of D (asuka_fock_build), the DIIS
+    push of F and e = FDS - (FDS)^T into a ring of diis_space slots with the
+    Gram row <e_i, e> updated in place, the extrapolation weights from
+    asuka_scf_diis_solve, the new density 2 X P X^T with P the purified
+    projector of X^T F X (_purify, in place of eigh + aufbau: the same
+    projector for a gapped spectrum, to the purification's idempotency
+    tolerance), the energy 1/2 sum D_new (h + F) + E_nuc, and the convergence
+    test |dE| < conv_tol and ||D_new - D||_F < conv_tol_dm, written to a
+    per-cycle record. The host replays check_every cycles, reads the record,
+    and stops at the first converged cycle, whose density and Fock matrix are
+    restored from a ring of the last check_every cycles. Orbitals and orbital
+    energies come from one eigh of the final Fock matrix."""
+    import ctypes
+
+    from .. import _lib
+
+    dev = S.device
+    nao = int(S.shape[0])
+    f64 = torch.float64
+    nslot = max(1, min(int(diis_space), 8)) if diis else 1
+    use_fock_out = torch.empty((nao, nao), dtype=f64, device=dev)
+
+    # purification steps: measured on the first Fock matrix, with a margin
+    F0 = direct_fock_rhf(jk_ctx, D0, h, mixed_precision=mixed_precision)
+    npur = int(1.5 * _purify_steps(torch, _sym(X.T @ F0 @ X), nocc)) + 8
+
+    st = {
+        "D": D0.clone(), "F": torch.empty((nao, nao), dtype=f64, device=dev),
+        "Fs": torch.zeros((8, nao, nao), dtype=f64, device=dev), "Es": torch.zeros((8, nao * nao), dtype=f64, device=dev),
+        "G": torch.zeros((8, 8), dtype=f64, device=dev), "count": torch.zeros(1, dtype=torch.int64, device=dev),
+        "coeff": torch.zeros(8, dtype=f64, device=dev), "status": torch.zeros(1, dtype=torch.int32, device=dev),
+        "E": torch.full((1,), float("nan"), dtype=f64, device=dev), "it": torch.zeros(1, dtype=torch.int64, device=dev),
+        "recE": torch.zeros(max_cycle + 1, dtype=f64, device=dev),
+        "recC": torch.zeros(max_cycle + 1, dtype=torch.bool, device=dev),
+        "recP": torch.zeros(max_cycle + 1, dtype=f64, device=dev),
+        "recD": torch.zeros(max_cycle + 1, dtype=f64, device=dev),
+        "ringD": torch.zeros((check_every, nao, nao), dtype=f64, device=dev),
+        "ringF": torch.zeros((check_every, nao, nao), dtype=f64, device=dev),
+    }
+    eye = torch.eye(nao, dtype=f64, device=dev)
+    L = _lib.lib()
+
+    def cycle():
+        D = st["D"]
+        F = direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed_precision, out=use_fock_out)
+        if level_shift:
+            F = X @ (X.T @ F @ X + level_shift * eye) @ X.T
+        if diis:
+            M = F @ D @ S
+            e = (M - M.T).reshape(-1)
+            slot = torch.remainder(st["count"], nslot)
+            st["Fs"].index_copy_(0, slot, F[None])
+            st["Es"].index_copy_(0, slot, e[None])
+            g = st["Es"] @ e
+            st["G"].index_copy_(0, slot, g[None])
+            st["G"].index_copy_(1, slot, g[:, None])
+            st["count"].add_(1)
+            _lib.check(L.asuka_scf_diis_solve(_lib.ptr(st["G"]), _lib.ptr(st["count"]), nslot, _lib.ptr(st["coeff"]),
+                                              _lib.ptr(st["status"]), ctypes.c_void_p(_lib.stream_handle(dev))))
+            F = _sym(torch.tensordot(st["coeff"], st["Fs"], dims=([0], [0])))
+        P, perr = _purify(torch, _sym(X.T @ F @ X), nocc, npur)
+        Dn = _sym(2.0 * (X @ P @ X.T))
+        if damping:
+            Dn = (1.0 - damping) * Dn + damping * D
+        E = 0.5 * (Dn * (h + F)).sum() + enuc
+        dm_err = torch.linalg.norm(Dn - D)
+        conv = ((E - st["E"]).abs() < conv_tol) & (dm_err < conv_tol_dm)
+        it = st["it"]
+        st["recE"].index_copy_(0, it, E.reshape(1))
+        st["recC"].index_copy_(0, it, conv.reshape(1))
+        st["recP"].index_copy_(0, it, perr.reshape(1))
+        st["recD"].index_copy_(0, it, dm_err.reshape(1))
+        ring = torch.remainder(it, check_every)
+        st["ringD"].index_copy_(0, ring, Dn[None])
+        st["ringF"].index_copy_(0, ring, F[None])
+        st["E"].copy_(E.reshape(1))
+        st["D"].copy_(Dn)
+        st["F"].copy_(F)
+        it.add_(1)
+
+    # warm the captured operations once outside the graph on a side stream
+    # (the allocator's pools and the library's lazy set-up), then restore the state
+    saved = {k: v.clone() for k, v in st.items()}
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        cycle()
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize(dev)
+    for k, v in saved.items():
+        st[k].copy_(v)
+    del saved
+    graph = None
+    if _GRAPH_REPLAY:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            cycle()
+    converged = False
+    ncyc = 0
+    first = None
+    while ncyc < max_cycle and first is None:
+        k = min(check_every, max_cycle - ncyc)
+        for _ in range(k):
+            if graph is not None:
+                graph.replay()
+            else:
+                cycle()
+        ncyc += k
+        rec = st["recC"][:ncyc].cpu().numpy()
+        hits = np.nonzero(rec)[0]
+        if hits.size:
+            first = int(hits[0])
+    if first is not None:
+        converged = True
+        ncyc = first + 1
+        r = first % check_every
+        D = st["ringD"][r].clone()
+        F = st["ringF"][r].clone()
+    else:
+        D = st["D"].clone()
+        F = st["F"].clone()
+    recE = st["recE"][:ncyc].cpu().numpy()
+    e_tot = float(recE[-1])
+    eps, C = _gen_eigh(torch, F, X)
+    if profile is not None:
+        profile.setdefault("scf", {})["iters"] = int(ncyc)
+        profile["scf"]["jk_calls"] = int(ncyc) + 1
+        profile["scf"]["graph"] = True
+        profile["scf"]["purify_steps"] = npur
+        profile["scf"]["purify_err_max"] = float(st["recP"][:ncyc].max())
+        profile["scf"]["energies"] = recE.tolist()
+        profile["scf"]["dm_errors"] = st["recD"][:ncyc].cpu().numpy().tolist()
+        profile["scf"]["purify_errors"] = st["recP"][:ncyc].cpu().numpy().tolist()
+    return SCFResult(method="RHF", converged=converged, niter=int(ncyc), e_tot=e_tot, e_elec=e_tot - enuc,
+                     e_nuc=enuc, mo_energy=eps, mo_coeff=C, mo_occ=occ, dm=D)
 
 
 def _occ_uhf(nalpha: int, nbeta: int, nao: int):

@@ -250,6 +250,43 @@ int asuka_jk_grad(asuka_jk_ctx* ctx, const double* D, const double* WJ, const do
  * asuka/mcscf/nuc_grad_direct.py:151-181. */
 int asuka_int1e_grad(asuka_jk_ctx* ctx, const double* Dh, const double* Ws, double* grad, void* stream);
 
+/* DIIS extrapolation weights on the device, for an SCF iteration captured in
+ * one HIP graph (no host round trip). G: 8 x 8 row-major Gram matrix
+ * <e_i, e_j> of the commutator errors kept in a ring of nslot (1..8) slots
+ * (slots 0..n-1 in use, n = min(*count, nslot)); count: device int64, vectors
+ * pushed so far (the newest in slot (*count - 1) % nslot). Writes coeff (8
+ * doubles, one weight per slot, zero for unused slots, summing to 1) and
+ * *status (1: extrapolated; 0: the newest slot alone, where the reference
+ * returns the newest Fock matrix). Same bordered system, scaling,
+ * regularisation ladder and renormalisation as the reference's
+ * DIIS.extrapolate (asuka/hf/df_scf.py:722-835); its least-squares retry,
+ * unreachable with a finite G, is not taken. Stream-ordered. */
+int asuka_scf_diis_solve(const double* G, const int64_t* count, int nslot, double* coeff, int32_t* status,
+                         void* stream);
+
+/* ---- Density fitting: auxiliary metric and three-centre integrals ---- */
+typedef struct asuka_df_ctx asuka_df_ctx;
+
+/* Shell and primitive-pair tables of an AO basis and an auxiliary basis (both
+ * in the packed layout of asuka_basis_desc, l <= 3; atoms unused), on the
+ * device. Replaces the plan/cache set-up of int3c2e_basis and
+ * metric_2c2e_basis (asuka/cueri/df.py:803-945). */
+int asuka_df_ctx_create(const asuka_basis_desc* ao, const asuka_basis_desc* aux, void* stream, asuka_df_ctx** out);
+int asuka_df_ctx_destroy(asuka_df_ctx* ctx);
+/* out[0]=nao, [1]=naux, [2]=AO shell pairs, [3]=auxiliary shells */
+int asuka_df_ctx_info(const asuka_df_ctx* ctx, int64_t out[4]);
+
+/* V (naux x naux, device, row-major, every element written) = (P|Q), the
+ * auxiliary Coulomb metric (asuka/cueri/df.py:803 metric_2c2e_basis).
+ * Stream-ordered. */
+int asuka_df_metric(asuka_df_ctx* ctx, double* V, void* stream);
+
+/* X (naux x nao x nao, device, row-major, every element written) =
+ * (mu nu|P) in the BQ layout X[P][mu][nu] of asuka/hf/df_jk.py:445
+ * (asuka/cueri/df.py:859 int3c2e_basis returns the same values as X(mu, nu, P)).
+ * Stream-ordered. */
+int asuka_df_int3c2e(asuka_df_ctx* ctx, double* X, void* stream);
+
 /* Rys roots (x = t^2, ascending) and weights of the n-point rule at T, from the
  * interpolation table the kernels use, evaluated on the host (n <= 7). */
 int asuka_rys_roots(int n, double T, double* x, double* w);

@@ -340,6 +340,99 @@ def direct_jk(ctx: OracleContext, D: np.ndarray, eps_schwarz: float = 1e-12, eps
     return direct_jk_tasks(ctx, D, ab, cd, hp=hp, threads=threads)
 
 
+# ---------------------------------------------------------------------------
+# Density fitting
+# ---------------------------------------------------------------------------
+
+
+def _unit_extended(ao, aux):
+    """One basis holding the AO shells, the auxiliary shells and, last, the
+    constant function 1 as an s shell of exponent 0 and coefficient 1, so that
+    (mu nu|P 1) = (mu nu|P) and (P 1|Q 1) = (P|Q) are four-centre tiles."""
+    from types import SimpleNamespace
+
+    nsa, nsx = int(ao.shell_l.size), int(aux.shell_l.size)
+    npa = int(ao.prim_exp.size)
+    ext = SimpleNamespace(
+        shell_cxyz=np.ascontiguousarray(np.vstack([ao.shell_cxyz, aux.shell_cxyz, np.zeros((1, 3))])),
+        shell_prim_start=np.ascontiguousarray(np.concatenate([ao.shell_prim_start, aux.shell_prim_start + npa,
+                                                              [npa + aux.prim_exp.size]]).astype(np.int32)),
+        shell_nprim=np.ascontiguousarray(np.concatenate([ao.shell_nprim, aux.shell_nprim, [1]]).astype(np.int32)),
+        shell_l=np.ascontiguousarray(np.concatenate([ao.shell_l, aux.shell_l, [0]]).astype(np.int32)),
+        shell_ao_start=np.ascontiguousarray(np.concatenate([ao.shell_ao_start, aux.shell_ao_start,
+                                                            [0]]).astype(np.int32)),
+        prim_exp=np.ascontiguousarray(np.concatenate([ao.prim_exp, aux.prim_exp, [0.0]])),
+        prim_coef=np.ascontiguousarray(np.concatenate([ao.prim_coef, aux.prim_coef, [1.0]])),
+        nshell=nsa + nsx + 1, nao=0)
+    return ext, nsa, nsx
+
+
+def df_integrals(ao, aux, hp: bool = True):
+    """(X, V): X[P][mu][nu] = (mu nu|P) and V[P][Q] = (P|Q) of the reference's
+    int3c2e_basis / metric_2c2e_basis (cueri/df.py:803-943), as oracle tiles
+    (_eri_rys_cpu.pyx:1096-1291) with the fourth function the constant 1."""
+    ext, nsa, nsx = _unit_extended(ao, aux)
+    ctx = make_context(ext)
+    unit = nsa + nsx
+    nc = lambda l: (l + 1) * (l + 2) // 2
+    nao = int(ao.shell_ao_start[-1]) + nc(int(ao.shell_l[-1]))
+    naux = int(aux.shell_ao_start[-1]) + nc(int(aux.shell_l[-1]))
+    pair_of = lambda i, j: (i * (i + 1) // 2 + j) if i >= j else (j * (j + 1) // 2 + i)
+    L = ext.shell_l
+    X = np.zeros((naux, nao, nao))
+    V = np.zeros((naux, naux))
+    for p in range(nsx):
+        pu = pair_of(unit, nsa + p)
+        p0 = int(aux.shell_ao_start[p])
+        for i in range(nsa):
+            for j in range(i + 1):
+                ab = pair_of(i, j)
+                A, B = int(ctx.sp_A[ab]), int(ctx.sp_B[ab])
+                t = eri_tile(ctx, ab, pu, hp=hp)  # (nA nB, nP)
+                a0, b0 = int(ext.shell_ao_start[A]), int(ext.shell_ao_start[B])
+                na, nb = nc(int(L[A])), nc(int(L[B]))
+                blk = t.reshape(na, nb, -1).transpose(2, 0, 1)
+                X[p0:p0 + blk.shape[0], a0:a0 + na, b0:b0 + nb] = blk
+                X[p0:p0 + blk.shape[0], b0:b0 + nb, a0:a0 + na] = blk.transpose(0, 2, 1)
+        for q in range(nsx):
+            t = eri_tile(ctx, pu, pair_of(unit, nsa + q), hp=hp)
+            q0 = int(aux.shell_ao_start[q])
+            V[p0:p0 + t.shape[0], q0:q0 + t.shape[1]] = t
+    return X, V
+
+
+def df_cholesky(V):
+    """Lower L with L L^T = V + s I, s from the reference's ladder
+    max(1e-14 max|V_ii|, 1e-12) x 10^k (cueri/df.py:946-991)."""
+    V = np.array(V, dtype=np.float64)
+    shift0 = max(float(np.abs(np.diag(V)).max()) * 1e-14, 1e-12)
+    added = 0.0
+    for k in range(6):
+        target = shift0 * 10.0 ** k
+        V[np.diag_indices_from(V)] += target - added
+        added = target
+        try:
+            L = np.linalg.cholesky(V)
+        except np.linalg.LinAlgError:
+            continue
+        if np.all(np.isfinite(L)):
+            return L
+    raise RuntimeError("cholesky failed")
+
+
+def df_JK(X, V, D):
+    """DF J and K (hf/df_jk.py:445-525) from B = L^-1 X (cueri/df.py:994):
+    J = sum_Q B_Q (B_Q . D), K = sum_Q B_Q D B_Q^T, symmetrised."""
+    import scipy.linalg
+
+    naux, nao, _ = X.shape
+    L = df_cholesky(0.5 * (V + V.T))
+    B = scipy.linalg.solve_triangular(L, X.reshape(naux, -1), lower=True).reshape(naux, nao, nao)
+    J = np.einsum("qmn,q->mn", B, np.einsum("qls,ls->q", B, D))
+    K = np.einsum("qml,ls,qns->mn", B, D, B, optimize=True)
+    return 0.5 * (J + J.T), 0.5 * (K + K.T)
+
+
 def dense_eri(ctx: OracleContext, hp: bool = True, threads: int = 0) -> np.ndarray:
     """eri_mat[pq, rs] (nao^2 x nao^2), cueri/dense_cpu.py:663-733 semantics."""
     nao = ctx.nao
