@@ -87,6 +87,7 @@ SYMBOLS = [
     ("asuka_jk_ctx_destroy", ctypes.c_int, [_vp]),
     ("asuka_jk_ctx_info", ctypes.c_int, [_vp, _i64p]),
     ("asuka_jk_build", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    ("asuka_jk_build_noalloc", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     ("asuka_scf_diis_solve", ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     ("asuka_df_ctx_create", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("asuka_df_ctx_destroy", ctypes.c_int, [_vp]),

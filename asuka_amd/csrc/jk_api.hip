@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1074,8 +1075,8 @@ static hipStream_t stream_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, h
   return (cp.stream > 0 && cp.stream <= (int)ctx->side.size()) ? ctx->side[cp.stream - 1] : s;
 }
 
-// queue: the class pairs' work-queue heads (the context's unless a caller
-// passes others); dev: the device tables the kernels read
+// queue: the class pairs' work-queue heads (the context's, or the static
+// scratch of asuka_jk_build_noalloc); dev: the device tables the kernels read
 static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool timed, unsigned int* queue = nullptr,
                        const DevBasis* dev = nullptr) {
   if (!queue) queue = ctx->d_queue;
@@ -1127,7 +1128,7 @@ static inline unsigned grid_for(long long n, int block) {
 // J/K of one density (D2 == nullptr) or of two densities from one evaluation
 // of each quartet; the outputs are zeroed first and symmetrised at the end.
 // queue, dsp (both or neither): per-build scratch in place of the context's
-// work-queue heads and shell-pair density maxima.
+// work-queue heads and shell-pair density maxima (asuka_jk_build_noalloc).
 static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, double* J, double* K, double* J2,
                          double* K2, bool wantJ, bool wantK, int flags, hipStream_t s, bool symmetrize = true,
                          unsigned int* queue = nullptr, double* dsp = nullptr) {
@@ -1216,6 +1217,59 @@ int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int
   const bool wantK = (flags & ASUKA_JK_WANT_K) && K;
   if (!wantJ && !wantK) return ASUKA_OK;
   return jk_build_impl(ctx, D, nullptr, J, K, nullptr, nullptr, wantJ, wantK, flags, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// Per-build scratch of asuka_jk_build_noalloc: the class pairs' work-queue
+// heads, then the shell-pair density maxima. One copy per GPU, kept until
+// the process ends; host code addresses it only through the pointer
+// hipGetSymbolAddress returns on that device.
+static constexpr size_t kNoAllocScratchBytes = 4u << 20;
+static constexpr size_t kNoAllocQueueBytes = 16u << 10;  // 4096 work-queue heads (2048 class pairs)
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int noalloc_scratch(unsigned char** out) {
+  static std::mutex mu;
+  static std::vector<unsigned char*> ptrs;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(ASUKA_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptrs[dev] = (unsigned char*)p;
+  }
+  *out = ptrs[dev];
+  return ASUKA_OK;
+}
+
+extern "C" {
+
+int asuka_jk_build_noalloc(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream) {
+  if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  if (reject_batch(ctx)) return ASUKA_E_INVALID;
+  const size_t nqueue = 2 * std::max<size_t>(1, ctx->cps.size());  // two heads per class pair
+  if (nqueue * sizeof(unsigned int) > kNoAllocQueueBytes)
+    return fail(ASUKA_E_INVALID, "asuka_jk_build_noalloc: " + std::to_string(ctx->cps.size()) +
+                                     " class pairs exceed the scratch limit of " +
+                                     std::to_string(kNoAllocQueueBytes / sizeof(unsigned int) / 2));
+  if ((size_t)ctx->nsp * sizeof(double) > kNoAllocScratchBytes - kNoAllocQueueBytes)
+    return fail(ASUKA_E_INVALID, "asuka_jk_build_noalloc: " + std::to_string(ctx->nsp) +
+                                     " shell pairs exceed the scratch limit of " +
+                                     std::to_string((kNoAllocScratchBytes - kNoAllocQueueBytes) / sizeof(double)));
+  if (flags & ASUKA_JK_TIME) return fail(ASUKA_E_INVALID, "asuka_jk_build_noalloc does not take ASUKA_JK_TIME");
+  const bool wantJ = (flags & ASUKA_JK_WANT_J) && J;
+  const bool wantK = (flags & ASUKA_JK_WANT_K) && K;
+  if (!wantJ && !wantK) return ASUKA_OK;
+  unsigned char* base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  return jk_build_impl(ctx, D, nullptr, J, K, nullptr, nullptr, wantJ, wantK, flags, (hipStream_t)stream, true,
+                       (unsigned int*)base, (double*)(base + kNoAllocQueueBytes));
 }
 
 int asuka_jk_build_multi(asuka_jk_ctx* ctx, const double* Da, const double* Db, double* Ja, double* Ka, double* Jb,
@@ -1330,12 +1384,12 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
   if (reps < 1) return fail(ASUKA_E_INVALID, "reps must be >= 1");
   hipStream_t s = (hipStream_t)stream;
   const long long nn = (long long)ctx->nao * ctx->nao;
-  double* scratch = nullptr;  // J and K of the timed launches (discarded)
-  HIP_CHECK(hipMalloc(&scratch, sizeof(double) * 2 * nn));
+  double* tjk = nullptr;  // J and K of the timed launches (discarded)
+  HIP_CHECK(hipMalloc(&tjk, sizeof(double) * 2 * nn));
   struct Free {
     double* p;
     ~Free() { (void)hipFree(p); }
-  } free_scratch{scratch};
+  } free_tjk{tjk};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
@@ -1351,8 +1405,8 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
   EriOut o{};
   o.mode = MODE_JK;
   o.D = D;
-  o.J = scratch;
-  o.K = scratch + nn;
+  o.J = tjk;
+  o.K = tjk + nn;
   o.eps_density = ctx->eps_density;
   o.eps_prim = ctx->eps_schwarz > 0.0 ? kPrimQuartetEps : 0.0;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;

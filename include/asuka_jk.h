@@ -107,6 +107,17 @@ int asuka_jk_ctx_info(const asuka_jk_ctx* ctx, int64_t out[8]);
  * (asuka/hf/direct_jk.py:1530-2202). J or K may be NULL when not wanted. */
 int asuka_jk_build(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream);
 
+/* asuka_jk_build with its per-build scratch (the class pairs' work-queue heads
+ * and the shell-pair density maxima) in a static device array of 4 MiB per GPU
+ * instead of the context's buffers: no device allocation, ever. No counterpart
+ * in the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). Limits:
+ * at most 2048 class pairs and 522240 shell pairs (larger contexts return
+ * ASUKA_E_INVALID); ASUKA_JK_TIME is rejected. The array is shared by every
+ * call on one GPU and keeps the last call's contents: calls that overlap on
+ * one GPU (any context, any stream) race. */
+int asuka_jk_build_noalloc(asuka_jk_ctx* ctx, const double* D, double* J, double* K, int flags, void* stream);
+
 /* (Ja, Ka, Jb, Kb) for two symmetric densities (UHF/ROHF alpha and beta)
  * from one evaluation of each shell quartet, digested with Da into (Ja, Ka)
  * and with Db into (Jb, Kb); screening uses max(|Da|, |Db|) per shell-pair
