@@ -10,6 +10,7 @@
 // pair and the length of the Q-descending prefix of the ket class list it
 // pairs with. Kernels decode quartets from (row, column) on the fly.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -94,6 +95,7 @@ struct ClassPairHost {
   long long ntasks0 = 0;
   std::vector<std::pair<int, std::pair<RowTask*, long long>>> cuts;  // other cuts made so far
   double tuned_ms = 0.0;                  // time of the class pair's launch when tuned (0: not tuned)
+  long long rtask_off = 0;                // first of its remapped row tasks (routed mixed builds)
 };
 
 // Row tasks of a class pair: each row's wave units cut into
@@ -187,6 +189,58 @@ __global__ void k_sym_copy(double* out, const double* X, int n) {
   }
 }
 
+// Routing of a mixed build, one wavefront per wave unit: f32[u] = 1 when the
+// unit has an active quartet and every active quartet's bound Q_ab Q_cd
+// max(D_ab, D_cd) is below tau (the FP32 kernel's vote), f64[u] = 1 when it
+// has an active quartet otherwise; both 0 when density screening removes every
+// quartet of the unit.
+__global__ void k_route_units(DevBasis b, const WaveUnit* __restrict__ units, const int* __restrict__ col_pair,
+                              long long nunits, double eps_density, double tau, int* f64, int* f32) {
+  const int lane = threadIdx.x & 63;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long u = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6; u < nunits; u += nw) {
+    const WaveUnit un = units[u];
+    bool active = lane < un.ncol;
+    double bound = 0.0;
+    if (active) {
+      const int Qp = col_pair[un.col + lane];
+      bound = (b.Q[un.P] * b.Q[Qp]) * fmax(b.Dsp[un.P], b.Dsp[Qp]);
+      if (eps_density > 0.0) active = bound >= eps_density;
+    }
+    const bool any = __ballot(active) != 0ull;
+    const bool lowp = __ballot(active && !(bound < tau)) == 0ull;
+    if (lane == 0) {
+      f32[u] = (any && lowp) ? 1 : 0;
+      f64[u] = (any && !lowp) ? 1 : 0;
+    }
+  }
+}
+
+// The routed units, in their original order, into the FP64 and FP32 lists
+// (pos: exclusive sums of the flags).
+__global__ void k_scatter_units(const WaveUnit* __restrict__ units, long long nunits, const int* __restrict__ f64,
+                                const int* __restrict__ f32, const int* __restrict__ pos64,
+                                const int* __restrict__ pos32, WaveUnit* u64, WaveUnit* u32) {
+  for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < nunits; u += (long long)gridDim.x * blockDim.x) {
+    if (f32[u]) u32[pos32[u]] = units[u];
+    if (f64[u]) u64[pos64[u]] = units[u];
+  }
+}
+
+// A class pair's row tasks (units [ubase + u0, ubase + u0 + nu) of the full
+// list) as the ranges their units occupy in the two routed lists; a range may
+// be empty.
+__global__ void k_route_tasks(const RowTask* __restrict__ tasks, long long ntasks, long long ubase,
+                              const int* __restrict__ pos64, const int* __restrict__ pos32, RowTask* t64,
+                              RowTask* t32) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < ntasks; t += (long long)gridDim.x * blockDim.x) {
+    const RowTask tk = tasks[t];
+    const long long g0 = ubase + tk.u0, g1 = g0 + tk.nu;
+    t64[t] = RowTask{pos64[g0], pos64[g1] - pos64[g0]};
+    t32[t] = RowTask{pos32[g0], pos32[g1] - pos32[g0]};
+  }
+}
+
 __global__ void k_rys_points(int n, long long npts, const double* T, double* x, double* w) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= npts) return;
@@ -236,6 +290,7 @@ struct asuka_jk_ctx {
   std::vector<double> shell_xyz, Q, atom_xyz, atom_Z;
   std::vector<PrimPair> pp;
   std::vector<PrimPairF> ppf;
+  PrimPairF32* d_ppf32 = nullptr;  // float copy of the final ppf (FP32 path)
   std::vector<ClassPairHost> cps;
   long long ntasks = 0;
   long long nprim_dropped = 0;  // primitive pairs removed by primitive-pair screening
@@ -247,6 +302,16 @@ struct asuka_jk_ctx {
   WaveUnit* d_units = nullptr;
   RowTask* d_tasks = nullptr;
   std::vector<WaveUnit> h_units;  // host copies of the wave units and ket lists
+  // routing of mixed builds: per wave unit FP64/FP32 flags (nunits + 1
+  // entries, the last 0), their exclusive sums, the two routed unit lists, the
+  // remapped row tasks of every class pair (rtask_cap entries each) and the
+  // scan scratch
+  int *d_rf64 = nullptr, *d_rf32 = nullptr, *d_rp64 = nullptr, *d_rp32 = nullptr;
+  WaveUnit *d_units64 = nullptr, *d_units32 = nullptr;
+  RowTask *d_rtasks64 = nullptr, *d_rtasks32 = nullptr;
+  long long rtask_cap = 0;
+  void* d_scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
   std::vector<int> h_col;
   int lao_off[ASUKA_LMAX + 2] = {0};
   size_t lds_per_group = 64 * 1024;  // device limit of LDS per workgroup
@@ -310,6 +375,51 @@ static bool row_acc_fits(const asuka_jk_ctx* ctx, int la, int lb, int lc, int ld
       sizeof(double) * (size_t)(ncart_c(la) * ncart_c(lb) + (ncart_c(la) + ncart_c(lb)) * (*cC + *cD));
   const int nr = (la + lb + lc + ld) / 2 + 1;
   return bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - root_table_lds(nr) - 1024);
+}
+
+// Offsets of each class pair's remapped row tasks (of its current cut) in the
+// routed task arrays, which grow when a cut needs more room (context creation
+// and the autotuner's re-cuts; never during a build).
+static int route_task_layout(asuka_jk_ctx* ctx) {
+  long long tot = 0;
+  for (auto& cp : ctx->cps) {
+    cp.rtask_off = tot;
+    tot += cp.ntasks;
+  }
+  tot = std::max<long long>(1, tot);
+  if (tot > ctx->rtask_cap) {
+    RowTask *a = nullptr, *b = nullptr;
+    HIP_CHECK(hipMalloc(&a, sizeof(RowTask) * (size_t)tot));
+    ctx->owned.push_back(a);
+    HIP_CHECK(hipMalloc(&b, sizeof(RowTask) * (size_t)tot));
+    ctx->owned.push_back(b);
+    ctx->d_rtasks64 = a;
+    ctx->d_rtasks32 = b;
+    ctx->rtask_cap = tot;
+  }
+  return ASUKA_OK;
+}
+
+// Device buffers of the routing of mixed builds (k_route_units and after).
+static int route_alloc(asuka_jk_ctx* ctx) {
+  const long long n = (long long)ctx->h_units.size();
+  const size_t ni = sizeof(int) * (size_t)(n + 1);
+  int** flags[4] = {&ctx->d_rf64, &ctx->d_rf32, &ctx->d_rp64, &ctx->d_rp32};
+  for (int** f : flags) {
+    HIP_CHECK(hipMalloc((void**)f, ni));
+    ctx->owned.push_back(*f);
+    HIP_CHECK(hipMemset(*f, 0, ni));  // context creation, not a build: the flags' last entry stays 0
+  }
+  const size_t nu = sizeof(WaveUnit) * (size_t)std::max<long long>(1, n);
+  HIP_CHECK(hipMalloc((void**)&ctx->d_units64, nu));
+  ctx->owned.push_back(ctx->d_units64);
+  HIP_CHECK(hipMalloc((void**)&ctx->d_units32, nu));
+  ctx->owned.push_back(ctx->d_units32);
+  HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, ctx->scan_tmp_bytes, ctx->d_rf64, ctx->d_rp64, (int)(n + 1)));
+  ctx->scan_tmp_bytes = std::max<size_t>(ctx->scan_tmp_bytes, 256);
+  HIP_CHECK(hipMalloc(&ctx->d_scan_tmp, ctx->scan_tmp_bytes));
+  ctx->owned.push_back(ctx->d_scan_tmp);
+  return route_task_layout(ctx);
 }
 
 extern "C" {
@@ -637,6 +747,26 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     }
   }
 
+  {  // the FP32 path's copy of the final primitive-pair table
+    std::vector<PrimPairF32> f32(ctx->ppf.size());
+    for (size_t t = 0; t < ctx->ppf.size(); ++t) {
+      const PrimPairF& f = ctx->ppf[t];
+      PrimPairF32& g = f32[t];
+      g.Px = f.Px;
+      g.Py = f.Py;
+      g.Pz = f.Pz;
+      g.eta = (float)f.eta;
+      g.inv_eta = (float)f.inv_eta;
+      g.ck = (float)f.ck;
+      g.PAx = (float)f.PAx;
+      g.PAy = (float)f.PAy;
+      g.PAz = (float)f.PAz;
+      g.q = (float)std::min(f.q, 3.0e38);
+      g.pad = 0.0f;
+    }
+    if ((rc = dev_upload(&ctx->d_ppf32, f32, own, s))) return rc;
+    dv.ppf32 = ctx->d_ppf32;
+  }
   if (ctx->pp_count0.empty()) ctx->pp_count0 = ctx->pp_count;
   {
     int* d_ppc0;
@@ -891,6 +1021,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     std::vector<unsigned int> qz(2 * std::max<size_t>(1, ctx->cps.size()), 0u);
     if ((rc = dev_upload(&ctx->d_queue, qz, own, s))) return rc;
   }
+  if ((rc = route_alloc(ctx))) return rc;
   HIP_CHECK(hipStreamSynchronize(s));
   *out = guard.release();
   return ASUKA_OK;
@@ -1053,6 +1184,11 @@ static ClassWork work_of(const asuka_jk_ctx* ctx, const ClassPairHost& cp, int i
   return w;
 }
 
+static inline unsigned grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  return (unsigned)std::max<long long>(1, std::min<long long>(g, 65535));
+}
+
 // Fork: the context's side streams wait for everything issued on s so far.
 static int fork_streams(asuka_jk_ctx* ctx, hipStream_t s) {
   if (ctx->side.empty()) return ASUKA_OK;
@@ -1092,6 +1228,24 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, queue, nq);
     HIP_CHECK(hipGetLastError());
   }
+  // Mixed builds: the precision vote of every wave unit once, before the
+  // launches, and the units split (in order) into an FP64 and an FP32 list, so
+  // each kernel of a class pair walks only its own units.
+  const long long nu_all = (long long)ctx->h_units.size();
+  const bool routed = o.mixed && ctx->d_rf64 && nu_all > 0;
+  if (routed) {
+    const long long nb = std::min<long long>((nu_all + 3) / 4, 65535);
+    hipLaunchKernelGGL(k_route_units, dim3((unsigned)nb), dim3(256), 0, s, *dev, ctx->d_units, ctx->d_col_pair,
+                       nu_all, o.eps_density, o.tau_mixed, ctx->d_rf64, ctx->d_rf32);
+    HIP_CHECK(hipGetLastError());
+    size_t tb = ctx->scan_tmp_bytes;
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tb, ctx->d_rf64, ctx->d_rp64, (int)(nu_all + 1), s));
+    tb = ctx->scan_tmp_bytes;
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tb, ctx->d_rf32, ctx->d_rp32, (int)(nu_all + 1), s));
+    hipLaunchKernelGGL(k_scatter_units, dim3(grid_for(nu_all, 256)), dim3(256), 0, s, ctx->d_units, nu_all,
+                       ctx->d_rf64, ctx->d_rf32, ctx->d_rp64, ctx->d_rp32, ctx->d_units64, ctx->d_units32);
+    HIP_CHECK(hipGetLastError());
+  }
   int rc = fork_streams(ctx, s);
   if (rc) return rc;
   long long n = 0;
@@ -1101,6 +1255,18 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
     ClassWork w = work_of(ctx, cph, i);
     w.queue = queue + i;
     w.queue32 = queue + ncp + i;
+    if (routed && w.ntasks > 0 && eri_fast_has_plan(w.la, w.lb, w.lc, w.ld)) {
+      RowTask* t64 = ctx->d_rtasks64 + cph.rtask_off;
+      RowTask* t32 = ctx->d_rtasks32 + cph.rtask_off;
+      hipLaunchKernelGGL(k_route_tasks, dim3(grid_for(w.ntasks, 256)), dim3(256), 0, ls, w.tasks, w.ntasks,
+                         cph.unit_start, ctx->d_rp64, ctx->d_rp32, t64, t32);
+      HIP_CHECK(hipGetLastError());
+      w.routed = 1;
+      w.units = ctx->d_units64;
+      w.tasks = t64;
+      w.units32 = ctx->d_units32;
+      w.tasks32 = t32;
+    }
     if (o.D2 && w.acc_colsC > 0) {  // two accumulator sets must fit the workgroup's LDS
       const size_t nA = ncart_c(w.la), nB = ncart_c(w.lb);
       const size_t bytes = 2 * sizeof(double) * (nA * nB + (nA + nB) * (size_t)(w.acc_colsC + w.acc_colsD));
@@ -1119,10 +1285,6 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
   return ASUKA_OK;
 }
 
-static inline unsigned grid_for(long long n, int block) {
-  long long g = (n + block - 1) / block;
-  return (unsigned)std::max<long long>(1, std::min<long long>(g, 65535));
-}
 
 
 // J/K of one density (D2 == nullptr) or of two densities from one evaluation
@@ -1464,7 +1626,7 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
     cp.tuned_ms = std::max(best, 1e-6);
   }
   assign_streams(ctx);
-  return ASUKA_OK;
+  return route_task_layout(ctx);  // the cuts may have changed
 }
 
 int asuka_jk_tuning_get(const asuka_jk_ctx* ctx, int32_t* rows, int cap) {
@@ -1504,7 +1666,7 @@ int asuka_jk_tuning_set(asuka_jk_ctx* ctx, const int32_t* rows, int n, void* str
     }
   }
   assign_streams(ctx);
-  return ASUKA_OK;
+  return route_task_layout(ctx);  // the cuts may have changed
 }
 
 int asuka_jk_last_timing(asuka_jk_ctx* ctx, double* ms, int64_t* nlaunch) {

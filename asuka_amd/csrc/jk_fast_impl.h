@@ -19,6 +19,8 @@
 // All component indices are compile-time (constexpr tables + full unrolling),
 // so the tables and accumulators live in VGPRs.
 #pragma once
+#include <type_traits>
+
 #include "jk_internal.h"
 #include "rys_eval.h"
 
@@ -252,7 +254,14 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   const double Cx = b.shell_xyz[3 * C], Cy = b.shell_xyz[3 * C + 1], Cz = b.shell_xyz[3 * C + 2];
   const int ab0 = uni(b.pp_start[P]);
   const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
-  const PrimPairF* __restrict__ ppf = b.ppf;
+  // the primitive pairs in the precision of CT (the FP32 path reads the float
+  // copy: no conversions in the primitive loop but P - Q, formed in FP64)
+  using PPT = typename std::conditional<sizeof(CT) == 4, PrimPairF32, PrimPairF>::type;
+  const PPT* __restrict__ ppf;
+  if constexpr (sizeof(CT) == 4)
+    ppf = b.ppf32;
+  else
+    ppf = b.ppf;
 
   CT ef[NEL * NFL];
 #pragma unroll
@@ -264,17 +273,17 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   // ket-pair bound of the unit is below eps_prim; a lane leaves its ket loop
   // at the first ket pair whose q times the largest bra bound is below it.
   // Every skipped primitive quartet adds less than eps_prim to each integral.
-  const double kcut = (nab > 0 && o.eps_prim > 0.0) ? o.eps_prim / ppf[ab0].q : 0.0;
-  PrimPairF Yn;
+  const CT kcut = (nab > 0 && o.eps_prim > 0.0) ? CT(o.eps_prim / (double)ppf[ab0].q) : CT(0);
+  PPT Yn;
   if (ncd > 0) Yn = ppf[cd0];
   for (int j = 0; j < ncd; ++j) {
-    const PrimPairF Y = Yn;
+    const PPT Y = Yn;
     if (Y.q < kcut) break;  // per lane: the rest of its ket pairs are smaller
     if (j + 1 < ncd) Yn = ppf[cd0 + j + 1];  // next ket pair, in flight during this one
     const CT q = CT(Y.eta), iq = CT(Y.inv_eta), ckq = CT(Y.ck);
     const CT QCx = CT(Y.PAx), QCy = CT(Y.PAy), QCz = CT(Y.PAz);
     for (int i = 0; i < nab; ++i) {
-      const PrimPairF X = ppf[ab0 + i];  // wave-uniform
+      const PPT X = ppf[ab0 + i];  // wave-uniform
       const CT p = CT(X.eta), ip = CT(X.inv_eta);
       const CT PAx = CT(X.PAx), PAy = CT(X.PAy), PAz = CT(X.PAz);
       const CT spq = p + q;
@@ -630,6 +639,7 @@ __device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const 
 template <class PL, typename CT>
 __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o,
                                                        const PrimPairF* __restrict__ ppf,
+                                                       const PrimPairF32* __restrict__ ppf32,
                                                        const WaveUnit* __restrict__ units,
                                                        const int* __restrict__ col_pair,
                                                        const int* __restrict__ sp_A, const int* __restrict__ sp_B,
@@ -641,6 +651,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
                                                        const double* __restrict__ Dsp,
                                                        const double* __restrict__ Dm) {
   b.ppf = ppf;
+  b.ppf32 = ppf32;
   b.sp_A = sp_A;
   b.sp_B = sp_B;
   b.pp_start = pp_start;
@@ -706,8 +717,9 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
     __syncthreads();
     const int t = s_task;
     if (t >= w.ntasks) break;  // uniform per workgroup; every workgroup gets here
-    const RowTask tk = w.tasks[t];
-    const int P = uni(w.units[tk.u0].P);
+    const RowTask tk = w.tasks[t];  // routed lists hold empty tasks (nu = 0): no unit to read
+    int P = 0;
+    if (tk.nu > 0) P = uni(w.units[tk.u0].P);
     bool worked = false;  // this wave digested an item of the task (uniform)
     // a wave item is one chunk of a unit, or for CARRY plans one C chunk with
     // its A chunks in sequence; items chunk-major: the waves of a workgroup
@@ -726,8 +738,10 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
         if (o.eps_density > 0.0) active = bound >= o.eps_density;
       }
       if (!wave_any(active)) continue;  // the whole unit is density-screened
-      const bool lowp = o.mixed && wave_all(!active || bound < o.tau_mixed);
-      if (lowp != kF32) continue;  // the other precision's kernel takes this unit
+      if (!w.routed) {  // without routed lists each kernel takes the units its vote gives it
+        const bool lowp = o.mixed && wave_all(!active || bound < o.tau_mixed);
+        if (lowp != kF32) continue;  // the other precision's kernel takes this unit
+      }
       worked = true;
       if (o.counters && chunk == 0) {
         // statistics: one atomic per counter and wave
@@ -813,11 +827,13 @@ inline size_t row_acc_bytes(int la, int lb, int colsC, int colsD) {
   return sizeof(double) * (size_t)(NA * NB + (NA + NB) * (colsC + colsD));
 }
 
-// The FP32 kernel's plan of a class: the FP64 plan's component chunks, or
-// (built with ASUKA_F32_HALVE) half as many, the FP32 intermediates taking
-// half the registers: A chunks halved when their count is even, else C chunks.
+// The FP32 kernel's plan of a class: half the FP64 plan's component chunks
+// (the FP32 intermediates take half the registers), A chunks halved when
+// their count is even, else C chunks; ASUKA_F32_HALVE=0 keeps the FP64 plan's.
+// Measured at (H2O)64 cc-pVDZ, single stream: (21|11) 9.6 -> 7.6 ms, (20|11)
+// 28.3 -> 18.3 ms, (21|21) 18.0 -> 11.7 ms per build for the FP32 kernels.
 #ifndef ASUKA_F32_HALVE
-#define ASUKA_F32_HALVE 0
+#define ASUKA_F32_HALVE 1
 #endif
 template <class PL>
 struct F32PlanOf {
@@ -835,7 +851,7 @@ hipError_t launch_one(const DevBasis& b, const ClassWork& w, const EriOut& o, si
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((eri_fast_kernel<PL, CT>), dim3((unsigned)g), dim3(64 * w.wpg), lds, s, b, w, o, b.ppf,
+  hipLaunchKernelGGL((eri_fast_kernel<PL, CT>), dim3((unsigned)g), dim3(64 * w.wpg), lds, s, b, w, o, b.ppf, b.ppf32,
                      w.units, w.col_pair, b.sp_A, b.sp_B, b.pp_start, b.pp_count, b.shell_xyz, b.shell_ao, b.Q,
                      b.Dsp, o.D);
   return hipGetLastError();
@@ -853,7 +869,12 @@ hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, h
       w.acc_colsC > 0 ? (o.D2 ? 2 : 1) * row_acc_bytes(PL::la, PL::lb, w.acc_colsC, w.acc_colsD) : 0;
   hipError_t e = launch_one<PL, double>(b, w, o, lds, g, s);
   if (e != hipSuccess || !o.mixed) return e;
-  return launch_one<typename F32PlanOf<PL>::type, float>(b, w, o, lds, g, s);
+  ClassWork w32 = w;
+  if (w.routed) {
+    w32.units = w.units32;
+    w32.tasks = w.tasks32;
+  }
+  return launch_one<typename F32PlanOf<PL>::type, float>(b, w32, o, lds, g, s);
 }
 
 using Launcher = hipError_t (*)(const DevBasis&, const ClassWork&, const EriOut&, hipStream_t);

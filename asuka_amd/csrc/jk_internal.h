@@ -37,6 +37,17 @@ struct __align__(16) PrimPairF {
   double q;              // Schwarz bound of the primitive pair alone (pairs sorted by it, descending)
 };
 
+// The primitive pair as the FP32 path of the class-specialised kernels reads
+// it: the same quantities rounded to float, except the centre P, kept in FP64
+// so that P - Q is formed in FP64 before rounding.
+struct __align__(16) PrimPairF32 {
+  double Px, Py, Pz;
+  float eta, inv_eta, ck;
+  float PAx, PAy, PAz;
+  float q;
+  float pad;
+};
+
 enum EriMode : int {
   MODE_JK = 0,       // accumulate J and/or K
   MODE_FOCK = 1,     // accumulate J - K/2 into F
@@ -58,6 +69,7 @@ struct DevBasis {
   const int* pp_count0;      // (nsp) primitive pairs before primitive-pair screening (work statistics)
   const PrimPair* pp;
   const PrimPairF* ppf;      // same indexing as pp
+  const PrimPairF32* ppf32;  // same indexing as pp
   const int* shell_lao;      // (nshell) first AO of the shell in the compact numbering of its l
   const int* lao_ao;         // compact (l, index) -> AO, concatenated over l
   const int* shell_mol;      // (nshell) molecule of the shell (0 unless a batch context)
@@ -131,6 +143,12 @@ struct ClassWork {
   long long q_nket;
   int q_bra0, q_ket0;
   int cp_index;  // class-pair index (statistics counters)
+  // mixed builds with routed units (the vote taken once per build before the
+  // launches): the FP32 kernel's units and row tasks; units and tasks above
+  // then hold the FP64 kernel's. routed == 0: each kernel votes in place.
+  const WaveUnit* units32;
+  const RowTask* tasks32;
+  int routed;
 };
 
 struct EriOut {
