@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -38,8 +39,11 @@ static constexpr double kDefaultTauMixed = 1e-7;
 static constexpr int kTaskUnits = 32;
 static constexpr int kTaskItems = 256;  // wave items (unit x chunk) per row task, for split plans
 // LDS accumulators of one row task larger than this (or than the device allows
-// next to the root tables) fall back to global atomics.
-static constexpr size_t kRowAccMaxBytes = 72 * 1024;
+// next to the root tables) fall back to global atomics; ASUKA_ROW_ACC_MAX_KB
+// lowers it. gfx950 gives a workgroup up to 160 KB: accumulators above 80 KB
+// leave one workgroup per CU (8 wavefronts), which the register kernels of
+// the classes that need them (2 wavefronts per SIMD by registers) fill anyway.
+static constexpr size_t kRowAccMaxBytes = 160 * 1024;
 // streams a build's class-pair launches are spread over (the caller's included)
 static constexpr int kDefaultStreams = 4;
 static constexpr int kMaxStreams = 8;
@@ -315,6 +319,7 @@ struct asuka_jk_ctx {
   std::vector<int> h_col;
   int lao_off[ASUKA_LMAX + 2] = {0};
   size_t lds_per_group = 64 * 1024;  // device limit of LDS per workgroup
+  size_t row_acc_max = kRowAccMaxBytes;  // LDS row-accumulator budget (bytes)
   int nlao[ASUKA_LMAX + 1] = {0};
   double* d_Q = nullptr;
   double* d_Dsp = nullptr;
@@ -374,7 +379,7 @@ static bool row_acc_fits(const asuka_jk_ctx* ctx, int la, int lb, int lc, int ld
   const size_t bytes =
       sizeof(double) * (size_t)(ncart_c(la) * ncart_c(lb) + (ncart_c(la) + ncart_c(lb)) * (*cC + *cD));
   const int nr = (la + lb + lc + ld) / 2 + 1;
-  return bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - root_table_lds(nr) - 1024);
+  return bytes <= std::min(ctx->row_acc_max, ctx->lds_per_group - root_table_lds(nr) - 1024);
 }
 
 // Offsets of each class pair's remapped row tasks (of its current cut) in the
@@ -451,6 +456,10 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
       ctx->lds_per_group = (size_t)v;
+    if (const char* e = std::getenv("ASUKA_ROW_ACC_MAX_KB"); e && *e) {
+      const long kb = std::strtol(e, nullptr, 10);
+      if (kb >= 0) ctx->row_acc_max = std::min(kRowAccMaxBytes, (size_t)kb * 1024);
+    }
   }
   ctx->nshell = nshell;
   ctx->shell_l.assign(bd->shell_l, bd->shell_l + nshell);
@@ -888,7 +897,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
         const int cC = ctx->nlao[cp.lc], cD = (cp.lc == cp.ld) ? 0 : ctx->nlao[cp.ld];
         const size_t bytes = sizeof(double) * (size_t)(ncart_c(cp.la) * ncart_c(cp.lb) +
                                                        (ncart_c(cp.la) + ncart_c(cp.lb)) * (cC + cD));
-        const bool fits = bytes <= std::min(kRowAccMaxBytes, ctx->lds_per_group - root_table_lds(nroots) - 1024);
+        const bool fits = bytes <= std::min(ctx->row_acc_max, ctx->lds_per_group - root_table_lds(nroots) - 1024);
         cp.acc_colsC = fits ? cC : 0;
         cp.acc_colsD = fits ? cD : 0;
         cp.wpg = (fits && bytes > kRowAccSmallBytes) ? 8 : 4;

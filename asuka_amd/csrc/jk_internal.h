@@ -13,7 +13,10 @@
 
 // The register kernels copy the root tables of up to this many roots into
 // LDS (each workgroup); larger root counts read them from global memory.
-constexpr int kLdsRootsMax = 4;
+#ifndef ASUKA_LDS_ROOTS_MAX
+#define ASUKA_LDS_ROOTS_MAX 4
+#endif
+constexpr int kLdsRootsMax = ASUKA_LDS_ROOTS_MAX;
 
 // One primitive pair of a shell pair (only pairs with a nonzero coefficient
 // product are kept; dropping exact zeros leaves every integral unchanged).
