@@ -183,6 +183,61 @@ __global__ void k_dsp(const double* D, const double* D2, DevBasis b, double* Dsp
   Dsp[sp] = m;
 }
 
+// Ket-position buffers (EriOut::jblk), one 64-thread workgroup per block:
+// Jpos zeroed and Dpos[.. c ..] = D[c-th component of the pair's block] for
+// the pair at each position (0 for padding); second density at offset len.
+__global__ void k_pos_gather(DevBasis b, const int* __restrict__ col, const long long* __restrict__ jblk,
+                             const double* __restrict__ D, const double* __restrict__ D2, double* Dpos,
+                             double* Jpos, long long len) {
+  const long long k = blockIdx.x;
+  const int lane = threadIdx.x;
+  const long long base = jblk[k];
+  const int ncd = (int)((jblk[k + 1] - base) >> 6);
+  const int sp = col[k * 64 + lane];
+  long long N = 0, msh = 0;
+  int aoC = 0, aoD = 0, nd = 1;
+  if (sp >= 0) {
+    const int C = b.sp_A[sp], Dh = b.sp_B[sp];
+    mol_layout(b, C, N, msh);
+    aoC = b.shell_ao[C];
+    aoD = b.shell_ao[Dh];
+    nd = ncart_c(b.shell_l[Dh]);
+  }
+  for (int c = 0; c < ncd; ++c) {
+    const long long i = base + 64LL * c + lane;
+    const long long e = msh + (aoC + c / nd) * N + aoD + c % nd;
+    Jpos[i] = 0.0;
+    Dpos[i] = sp >= 0 ? D[e] : 0.0;
+    if (D2) {
+      Jpos[len + i] = 0.0;
+      Dpos[len + i] = sp >= 0 ? D2[e] : 0.0;
+    }
+  }
+}
+
+// After the launches: J[c,d] += Jpos (and J2 += the second set). Each element
+// of J belongs to one ket position, and no other kernel of the build runs
+// any more, so the adds need no atomics.
+__global__ void k_pos_scatter(DevBasis b, const int* __restrict__ col, const long long* __restrict__ jblk,
+                              const double* __restrict__ Jpos, double* J, double* J2, long long len) {
+  const long long k = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int sp = col[k * 64 + lane];
+  if (sp < 0) return;
+  const long long base = jblk[k];
+  const int ncd = (int)((jblk[k + 1] - base) >> 6);
+  const int C = b.sp_A[sp], Dh = b.sp_B[sp];
+  long long N, msh;
+  mol_layout(b, C, N, msh);
+  const int aoC = b.shell_ao[C], aoD = b.shell_ao[Dh], nd = ncart_c(b.shell_l[Dh]);
+  for (int c = 0; c < ncd; ++c) {
+    const long long i = base + 64LL * c + lane;
+    const long long e = msh + (aoC + c / nd) * N + aoD + c % nd;
+    if (J) J[e] += Jpos[i];
+    if (J2) J2[e] += Jpos[len + i];
+  }
+}
+
 // out = (X + X^T) / 2 for an n x n block X (batch builds: the packed block of
 // one molecule to the caller's matrix).
 __global__ void k_sym_copy(double* out, const double* X, int n) {
@@ -316,6 +371,11 @@ struct asuka_jk_ctx {
   long long rtask_cap = 0;
   void* d_scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
+  // ket-position buffers (EriOut::jblk): nblk + 1 block offsets, then J[c,d]
+  // and D[c,d] for two densities (2 * pos_len doubles each)
+  long long nblk = 0, pos_len = 0;
+  long long* d_jblk = nullptr;
+  double *d_jpos = nullptr, *d_dpos = nullptr;
   std::vector<int> h_col;
   int lao_off[ASUKA_LMAX + 2] = {0};
   size_t lds_per_group = 64 * 1024;  // device limit of LDS per workgroup
@@ -406,6 +466,31 @@ static int route_task_layout(asuka_jk_ctx* ctx) {
 }
 
 // Device buffers of the routing of mixed builds (k_route_units and after).
+// Block offsets of the ket-position buffers (EriOut::jblk) for the padded ket
+// lists, and the buffers themselves.
+static int pos_layout(asuka_jk_ctx* ctx, const std::vector<int>& col) {
+  const long long nblk = (long long)col.size() / 64;
+  std::vector<long long> jblk((size_t)nblk + 1, 0);
+  for (long long k = 0; k < nblk; ++k) {
+    int ncd = 0;
+    for (int t = 0; t < 64 && !ncd; ++t) {
+      const int sp = col[(size_t)(k * 64 + t)];
+      if (sp >= 0) ncd = ncart_c(ctx->shell_l[ctx->sp_A[sp]]) * ncart_c(ctx->shell_l[ctx->sp_B[sp]]);
+    }
+    jblk[(size_t)k + 1] = jblk[(size_t)k] + 64LL * ncd;
+  }
+  ctx->nblk = nblk;
+  ctx->pos_len = jblk.back();
+  int rc = dev_upload(&ctx->d_jblk, jblk, ctx->owned, nullptr);
+  if (rc) return rc;
+  const size_t bytes = sizeof(double) * (size_t)std::max<long long>(1, 2 * ctx->pos_len);
+  HIP_CHECK(hipMalloc((void**)&ctx->d_jpos, bytes));
+  ctx->owned.push_back(ctx->d_jpos);
+  HIP_CHECK(hipMalloc((void**)&ctx->d_dpos, bytes));
+  ctx->owned.push_back(ctx->d_dpos);
+  return ASUKA_OK;
+}
+
 static int route_alloc(asuka_jk_ctx* ctx) {
   const long long n = (long long)ctx->h_units.size();
   const size_t ni = sizeof(int) * (size_t)(n + 1);
@@ -873,6 +958,9 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
         }
       g.kpref.assign(1, 0);
       for (int sp : g.pairs) g.kpref.push_back(g.kpref.back() + ctx->pp_count[sp]);
+      // every group starts a 64-position block (wave units then start at
+      // block boundaries, EriOut::jblk); -1 marks the padding
+      col_all.resize((col_all.size() + 63) / 64 * 64, -1);
       g.col_start = (int)col_all.size();
       col_all.insert(col_all.end(), g.pairs.begin(), g.pairs.end());
       groups[c].push_back(std::move(g));
@@ -1015,6 +1103,8 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     }
   }
   assign_streams(ctx);
+  col_all.resize((col_all.size() + 63) / 64 * 64, -1);
+  if ((rc = pos_layout(ctx, col_all))) return rc;
   if ((rc = dev_upload(&ctx->d_segs, segs, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_units, units, own, s))) return rc;
   if ((rc = dev_upload(&ctx->d_tasks, tasks, own, s))) return rc;
@@ -1255,6 +1345,24 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
                        ctx->d_rf64, ctx->d_rf32, ctx->d_rp64, ctx->d_rp32, ctx->d_units64, ctx->d_units32);
     HIP_CHECK(hipGetLastError());
   }
+  // J[c,d] and D[c,d] by ket position for the register kernels of J/K and
+  // Fock builds (EriOut::jblk)
+  EriOut oo = o;
+  double* Jt = (o.mode == MODE_FOCK) ? o.F : (o.mode == MODE_JK ? o.J : nullptr);
+  double* Jt2 = (o.mode == MODE_JK && o.D2) ? o.J2 : nullptr;
+  const bool pos = ctx->d_jblk && ctx->nblk > 0 && (Jt || Jt2);
+  if (pos) {
+    oo.jblk = ctx->d_jblk;
+    oo.Jpos = ctx->d_jpos;
+    oo.Dpos = ctx->d_dpos;
+    if (o.D2) {
+      oo.Jpos2 = ctx->d_jpos + ctx->pos_len;
+      oo.Dpos2 = ctx->d_dpos + ctx->pos_len;
+    }
+    hipLaunchKernelGGL(k_pos_gather, dim3((unsigned)ctx->nblk), dim3(64), 0, s, *dev, ctx->d_col_pair, ctx->d_jblk,
+                       o.D, o.D2, ctx->d_dpos, ctx->d_jpos, ctx->pos_len);
+    HIP_CHECK(hipGetLastError());
+  }
   int rc = fork_streams(ctx, s);
   if (rc) return rc;
   long long n = 0;
@@ -1283,12 +1391,17 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
         w.acc_colsC = w.acc_colsD = 0;
     }
     bool handled = false;
-    HIP_CHECK(launch_eri_fast(*dev, w, o, ls, &handled));
+    HIP_CHECK(launch_eri_fast(*dev, w, oo, ls, &handled));
     if (!handled) HIP_CHECK(launch_eri_coop(*dev, w, o, ls, &handled));
     if (!handled) HIP_CHECK(launch_eri_generic(*dev, w, o, ls));
     ++n;
   }
   if ((rc = join_streams(ctx, s))) return rc;
+  if (pos) {
+    hipLaunchKernelGGL(k_pos_scatter, dim3((unsigned)ctx->nblk), dim3(64), 0, s, *dev, ctx->d_col_pair, ctx->d_jblk,
+                       ctx->d_jpos, Jt, Jt2, ctx->pos_len);
+    HIP_CHECK(hipGetLastError());
+  }
   if (timed) HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
   ctx->last_launches = n;
   return ASUKA_OK;

@@ -231,9 +231,12 @@ struct RowAcc {
 // first density over this chunk's A components are added into jcd (the
 // lane's registers, NCL*ND values) and the caller issues one atomic per
 // element after the last A chunk; nullptr: atomics here.
+// jb: the unit's block in the ket-position buffers (EriOut::jblk; uniform),
+// or -1 for D[c,d] from D and J[c,d] atomics on J.
+
 template <class PL, int CH, typename CT>
 __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
-                                          const CT* __restrict__ tab, const RowAcc& acc, double* jcd) {
+                                          const CT* __restrict__ tab, const RowAcc& acc, double* jcd, long long jb) {
   using CK = Chunk<PL, CH>;
   constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
   constexpr int NMAX = PL::NMAX, MMAX = PL::MMAX;
@@ -424,16 +427,22 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
     double* Ko = shifted(di ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
     const int so = di * acc.stride;  // this density's accumulators in s_acc
     if (Jo) {
+      const int lane = threadIdx.x & 63;
       double dcd[NCDL];
+      if (jb >= 0) {  // the chunk's components c0 * ND.. of the lane's ket position
+        const double* Dp = (di ? o.Dpos2 : o.Dpos) + jb + 64LL * (c0 * ND) + lane;
 #pragma unroll
-      for (int ic = 0; ic < NCL; ++ic)
+        for (int c = 0; c < NCDL; ++c) dcd[c] = Dp[64 * c];
+      } else {
 #pragma unroll
-        for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = Dm[(aoC + ic) * N + aoD + id];
+        for (int ic = 0; ic < NCL; ++ic)
+#pragma unroll
+          for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = Dm[(aoC + ic) * N + aoD + id];
+      }
       // The bra pair, hence the J[a,b] addresses, is the same on every lane:
       // reduce across the wave, one update per element.
       const unsigned long long amask = __ballot(active);
       const int first = amask ? (__ffsll((unsigned long long)amask) - 1) : 0;
-      const int lane = threadIdx.x & 63;
 #pragma unroll
       for (int r = 0; r < NAL * NB; ++r) {
         double s = 0.0;
@@ -460,6 +469,8 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
           for (int r = 0; r < NAL * NB; ++r) s = fma((double)tile[r * NCDL + c], dab[r], s);
           if (jcd && di == 0)
             jcd[c] = fma(fj, s, jcd[c]);
+          else if (jb >= 0)
+            atomicAdd((di ? o.Jpos2 : o.Jpos) + jb + 64LL * (c0 * ND + c) + lane, fj * s);
           else
             atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
         }
@@ -557,23 +568,25 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 
 template <class PL, int CH, typename CT>
 __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp, int nab,
-                                               bool active, const CT* __restrict__ tab, const RowAcc& acc) {
+                                               bool active, const CT* __restrict__ tab, const RowAcc& acc,
+                                               long long jb) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH, CT>(b, o, P, Qp, nab, active, tab, acc, nullptr);
+      run_chunk<PL, CH, CT>(b, o, P, Qp, nab, active, tab, acc, nullptr, jb);
       return;
     }
-    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, nab, active, tab, acc);
+    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, nab, active, tab, acc, jb);
   }
 }
 
 // The A chunks CA.. of C chunk CC, in sequence, carrying J[c,d] in jcd.
 template <class PL, int CC, int CA, typename CT>
 __device__ __forceinline__ void run_agroup(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
-                                           const CT* __restrict__ tab, const RowAcc& acc, double* jcd) {
+                                           const CT* __restrict__ tab, const RowAcc& acc, double* jcd,
+                                           long long jb) {
   if constexpr (CA < PL::SPLIT_A) {
-    run_chunk<PL, CA * PL::SPLIT_C + CC, CT>(b, o, P, Qp, nab, active, tab, acc, jcd);
-    run_agroup<PL, CC, CA + 1, CT>(b, o, P, Qp, nab, active, tab, acc, jcd);
+    run_chunk<PL, CA * PL::SPLIT_C + CC, CT>(b, o, P, Qp, nab, active, tab, acc, jcd, jb);
+    run_agroup<PL, CC, CA + 1, CT>(b, o, P, Qp, nab, active, tab, acc, jcd, jb);
   }
 }
 
@@ -582,15 +595,19 @@ __device__ __forceinline__ void run_agroup(const DevBasis& b, const EriOut& o, i
 // receives one atomic per element and quartet instead of one per A chunk.
 template <class PL, int CC, typename CT>
 __device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
-                                          const CT* __restrict__ tab, const RowAcc& acc) {
+                                          const CT* __restrict__ tab, const RowAcc& acc, long long jb) {
   using CK = Chunk<PL, CC>;  // A chunk 0: the C range of every A chunk of CC
   constexpr int ND = PL::ND, NCDL = (CK::c1 - CK::c0) * ND;
   double jcd[NCDL];
 #pragma unroll
   for (int c = 0; c < NCDL; ++c) jcd[c] = 0.0;
-  run_agroup<PL, CC, 0, CT>(b, o, P, Qp, nab, active, tab, acc, jcd);
+  run_agroup<PL, CC, 0, CT>(b, o, P, Qp, nab, active, tab, acc, jcd, jb);
   double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
-  if (Jo && active && P != Qp) {
+  if (Jo && active && P != Qp && jb >= 0) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < NCDL; ++c) atomicAdd(o.Jpos + jb + 64LL * (CK::c0 * ND + c) + lane, jcd[c]);
+  } else if (Jo && active && P != Qp) {
     const int A = uni(b.sp_A[P]), C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
     long long N, msh;
     mol_layout(b, A, N, msh);
@@ -605,15 +622,16 @@ __device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, in
 // one chunk otherwise.
 template <class PL, int CC, typename CT>
 __device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const EriOut& o, int P, int Qp, int nab,
-                                               bool active, const CT* __restrict__ tab, const RowAcc& acc) {
+                                               bool active, const CT* __restrict__ tab, const RowAcc& acc,
+                                               long long jb) {
   if constexpr (!PL::CARRY) {
-    dispatch_chunk<PL, 0, CT>(cc, b, o, P, Qp, nab, active, tab, acc);
+    dispatch_chunk<PL, 0, CT>(cc, b, o, P, Qp, nab, active, tab, acc, jb);
   } else if constexpr (CC < PL::SPLIT_C) {
     if (cc == CC) {
-      run_citem<PL, CC, CT>(b, o, P, Qp, nab, active, tab, acc);
+      run_citem<PL, CC, CT>(b, o, P, Qp, nab, active, tab, acc, jb);
       return;
     }
-    dispatch_citem<PL, CC + 1, CT>(cc, b, o, P, Qp, nab, active, tab, acc);
+    dispatch_citem<PL, CC + 1, CT>(cc, b, o, P, Qp, nab, active, tab, acc, jb);
   }
 }
 
@@ -768,7 +786,9 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       // atomics so the wave stays converged for the cross-lane reductions
       // bra primitive pairs this unit evaluates (see run_chunk)
       const int nab = uni(o.eps_prim > 0.0 ? un.nab : b.pp_count[P]);
-      dispatch_citem<PL, 0, CT>(chunk, b, o, P, Qp, nab, active, tab, acc);
+      // the unit's block of the ket-position buffers (units start at block boundaries)
+      const long long jb = o.jblk ? o.jblk[un.col >> 6] : -1LL;
+      dispatch_citem<PL, 0, CT>(chunk, b, o, P, Qp, nab, active, tab, acc, jb);
     }
     if (use_acc) {
       if (worked && lane == 0) s_work = 1;

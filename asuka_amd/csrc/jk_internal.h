@@ -179,6 +179,18 @@ struct EriOut {
   // auxiliary function indices are the shells' shell_ao
   double* X;
   long long x_nao, x_naux;
+  // Ket-position buffers of the register kernels (null: J[c,d] by atomics on
+  // J and D[c,d] read from D). Ket lists (col_pair) are laid out in blocks of
+  // 64 positions of one ket class; block k holds component c (= ic * ND + id)
+  // of the pair at position 64 k + lane at jblk[k] + 64 c + lane. A wave
+  // unit starts at a block boundary, so its lanes read D[c,d] (Dpos, gathered
+  // before the launches) and add J[c,d] (Jpos, added into J after them) with
+  // contiguous addresses. Jpos2 / Dpos2: the second density's.
+  const long long* jblk;
+  double* Jpos;
+  double* Jpos2;
+  const double* Dpos;
+  const double* Dpos2;
 };
 
 constexpr int kCountStride = 16;
