@@ -238,6 +238,16 @@ __global__ void k_pos_scatter(DevBasis b, const int* __restrict__ col, const lon
   }
 }
 
+// X[i][j] += Xc[i][g(j)]: the permuted-column flush buffer (EriOut::Kc) into
+// the output, g(j) the compact column of AO j.
+__global__ void k_unperm_add(double* X, const double* __restrict__ Xc, const int* __restrict__ ao_g, int n) {
+  const long long nn = (long long)n * n;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nn; t += (long long)gridDim.x * blockDim.x) {
+    const long long i = t / n, j = t - i * n;
+    X[t] += Xc[i * n + ao_g[j]];
+  }
+}
+
 // out = (X + X^T) / 2 for an n x n block X (batch builds: the packed block of
 // one molecule to the caller's matrix).
 __global__ void k_sym_copy(double* out, const double* X, int n) {
@@ -376,6 +386,10 @@ struct asuka_jk_ctx {
   long long nblk = 0, pos_len = 0;
   long long* d_jblk = nullptr;
   double *d_jpos = nullptr, *d_dpos = nullptr;
+  // permuted-column K of the row-task flush (EriOut::Kc; one-molecule
+  // contexts): two nao x nao buffers and the compact column of each AO
+  double* d_kc = nullptr;
+  int* d_ao_g = nullptr;
   std::vector<int> h_col;
   int lao_off[ASUKA_LMAX + 2] = {0};
   size_t lds_per_group = 64 * 1024;  // device limit of LDS per workgroup
@@ -701,6 +715,13 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     ctx->dev.lao_ao = d_laoao;
     ctx->dev.mol_lao = d_mlao;
     ctx->dev.shell_mol = d_smol;
+    if (nmol == 1 && (int)lao_ao.size() == ctx->nao) {  // permuted-column K of the flush (EriOut::Kc)
+      std::vector<int> ao_g(ctx->nao);
+      for (int g = 0; g < ctx->nao; ++g) ao_g[lao_ao[g]] = g;
+      if ((rc = dev_upload(&ctx->d_ao_g, ao_g, own, s))) return rc;
+      HIP_CHECK(hipMalloc((void**)&ctx->d_kc, sizeof(double) * 2 * (size_t)ctx->nao * ctx->nao));
+      own.push_back(ctx->d_kc);
+    }
   }
   // per-molecule D/J/K blocks (a molecule's AOs are contiguous): row-major
   // n_m x n_m blocks packed one after another; one molecule: plain nao x nao
@@ -1363,6 +1384,18 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
                        o.D, o.D2, ctx->d_dpos, ctx->d_jpos, ctx->pos_len);
     HIP_CHECK(hipGetLastError());
   }
+  // K rows of the row-task flush with permuted columns (EriOut::Kc)
+  double* Kt = (o.mode == MODE_FOCK) ? o.F : (o.mode == MODE_JK ? o.K : nullptr);
+  double* Kt2 = (o.mode == MODE_JK && o.D2) ? o.K2 : nullptr;
+  const long long nn = (long long)ctx->nao * ctx->nao;
+  const bool perm = ctx->d_kc && (Kt || Kt2);
+  if (perm) {
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((Kt2 ? 2 : 1) * nn, 256)), dim3(256), 0, s, ctx->d_kc,
+                       (Kt2 ? 2 : 1) * nn, 0.0);
+    HIP_CHECK(hipGetLastError());
+    if (Kt) oo.Kc = ctx->d_kc;
+    if (Kt2) oo.Kc2 = ctx->d_kc + nn;
+  }
   int rc = fork_streams(ctx, s);
   if (rc) return rc;
   long long n = 0;
@@ -1400,6 +1433,13 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
   if (pos) {
     hipLaunchKernelGGL(k_pos_scatter, dim3((unsigned)ctx->nblk), dim3(64), 0, s, *dev, ctx->d_col_pair, ctx->d_jblk,
                        ctx->d_jpos, Jt, Jt2, ctx->pos_len);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (perm) {
+    if (Kt) hipLaunchKernelGGL(k_unperm_add, dim3(grid_for(nn, 256)), dim3(256), 0, s, Kt, ctx->d_kc, ctx->d_ao_g, ctx->nao);
+    if (Kt2)
+      hipLaunchKernelGGL(k_unperm_add, dim3(grid_for(nn, 256)), dim3(256), 0, s, Kt2, ctx->d_kc + nn, ctx->d_ao_g,
+                         ctx->nao);
     HIP_CHECK(hipGetLastError());
   }
   if (timed) HIP_CHECK(hipEventRecord(ctx->ev_stop, s));

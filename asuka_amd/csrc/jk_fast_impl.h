@@ -726,6 +726,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
   __syncthreads();
   const CT* tab = kLds ? tct : gct;
   const int lane = threadIdx.x & 63;
+
   unsigned int* queue = kF32 ? w.queue32 : w.queue;
   // launched with kGroupThreads-thread blocks; blockDim is not read because
   // the compiler cannot prove it uniform, which would make the loops divergent
@@ -803,6 +804,8 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
         const int* laoC = b.lao_ao + w.lao_offC + mlao[PL::lc];
         const int* laoD = b.lao_ao + (cD > 0 ? w.lao_offD + mlao[PL::ld] : w.lao_offC + mlao[PL::lc]);
         const int cDe = cD > 0 ? cD : cC;
+        // compact columns of the C and D lists (EriOut::Kc)
+        const int gC = w.lao_offC + mlao[PL::lc], gD = cD > 0 ? w.lao_offD + mlao[PL::ld] : gC;
         for (int t2 = threadIdx.x; t2 < nacc * nden; t2 += nthr) {
           const double v = s_acc[t2];
           if (v == 0.0) continue;
@@ -810,28 +813,33 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
           const bool second = t2 >= nacc;
           double* Jo = shifted(second ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
           double* Ko = shifted(second ? o.K2 : ((o.mode == MODE_FOCK) ? o.F : o.K), msh);
+          double* Kc = second ? o.Kc2 : o.Kc;  // uniform
           int r = second ? t2 - nacc : t2;
           if (r < NA * NB) {
             atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], v);
             continue;
           }
           r -= NA * NB;
+          long long row, k;  // the K row (AO) and the entry's index in its column list
+          int g;             // first compact column of that list
+          const int* lao;
           if (r < NA * cC) {
-            atomicAdd(&Ko[(aoA + r / cC) * N + laoC[r % cC]], v);
-            continue;
+            row = aoA + r / cC, k = r % cC, g = gC, lao = laoC;
+          } else if ((r -= NA * cC), cD > 0 && r < NA * cD) {
+            row = aoA + r / cD, k = r % cD, g = gD, lao = laoD;
+          } else {
+            if (cD > 0) r -= NA * cD;
+            if (r < NB * cC) {
+              row = aoB + r / cC, k = r % cC, g = gC, lao = laoC;
+            } else {
+              r -= NB * cC;
+              row = aoB + r / cDe, k = r % cDe, g = gD, lao = laoD;
+            }
           }
-          r -= NA * cC;
-          if (cD > 0 && r < NA * cD) {
-            atomicAdd(&Ko[(aoA + r / cD) * N + laoD[r % cD]], v);
-            continue;
-          }
-          if (cD > 0) r -= NA * cD;
-          if (r < NB * cC) {
-            atomicAdd(&Ko[(aoB + r / cC) * N + laoC[r % cC]], v);
-            continue;
-          }
-          r -= NB * cC;
-          atomicAdd(&Ko[(aoB + r / cDe) * N + laoD[r % cDe]], v);
+          if (Kc)  // consecutive entries, consecutive addresses
+            atomicAdd(&Kc[row * N + g + k], v);
+          else
+            atomicAdd(&Ko[row * N + lao[k]], v);
         }
       }
     }

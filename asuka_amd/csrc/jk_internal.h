@@ -191,6 +191,13 @@ struct EriOut {
   double* Jpos2;
   const double* Dpos;
   const double* Dpos2;
+  // K rows of the register kernels' row-task flush with permuted columns
+  // (one-molecule contexts; null: into K itself): column g holds AO
+  // lao_ao[g], so a row's columns of one angular momentum are contiguous
+  // and the flush's atomics coalesce; added into K (or F) after the
+  // launches. Kc2: the second density's.
+  double* Kc;
+  double* Kc2;
 };
 
 constexpr int kCountStride = 16;
