@@ -775,8 +775,25 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
         }
         const unsigned long long ksum = (unsigned long long)wave_sum((double)kq);
         const unsigned long long nact = (unsigned long long)__popcll(__ballot(active));
+        // primitive quartets the loops evaluate (after the primitive-quartet
+        // cut) and the lane slots the wave spends on them (64 x the longest lane)
+        const int nabc = uni(o.eps_prim > 0.0 ? un.nab : b.pp_count[P]);
+        long long nev = 0;
+        if (active && nabc > 0) {
+          const int cd0 = b.pp_start[Qp], ncd = b.pp_count[Qp];
+          const double kcut = o.eps_prim > 0.0 ? o.eps_prim / b.ppf[b.pp_start[P]].q : 0.0;
+          int j = 0;
+          while (j < ncd && !(b.ppf[cd0 + j].q < kcut)) ++j;
+          nev = (long long)nabc * j;
+        }
+        const double nevs = wave_sum((double)nev);
+        double nmax = (double)nev;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) nmax = fmax(nmax, __shfl_xor(nmax, off));
         if (lane == 0) {
           atomicAdd(cnt + 8, ksum);
+          atomicAdd(cnt + 11, (unsigned long long)nevs);
+          atomicAdd(cnt + 12, 64ull * (unsigned long long)nmax);
           if (kF32) {
             atomicAdd(cnt + 9, nact);
             atomicAdd(cnt + 10, ksum);

@@ -96,14 +96,17 @@ class DirectJKContext:
         :func:`direct_JK`): per class pair, in :meth:`class_stats` order, the
         quartets evaluated after density screening by symmetry bin
         (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair), their primitive
-        quartets, and the FP32 share of both."""
+        quartets, the FP32 share of both, and for the class-specialised kernels
+        the primitive quartets their loops evaluate and the lane slots those
+        loops take (their ratio is the loops' SIMD utilisation)."""
         cls = self.class_stats()
         n = len(cls)
         rows = np.zeros((max(1, n), 16), dtype=np.int64)
         _lib.check(_lib.lib().asuka_jk_last_counts(ctypes.c_void_p(self.handle), rows.ctypes.data_as(_lib._i64p), n,
                                                    ctypes.c_void_p(_lib.stream_handle(self.device))))
         return [{"class": c["class"], "bins": rows[i, :8].copy(), "prim_quartets": int(rows[i, 8]),
-                 "quartets_fp32": int(rows[i, 9]), "prim_quartets_fp32": int(rows[i, 10])}
+                 "quartets_fp32": int(rows[i, 9]), "prim_quartets_fp32": int(rows[i, 10]),
+                 "prim_quartets_evaluated": int(rows[i, 11]), "prim_lane_slots": int(rows[i, 12])}
                 for i, c in enumerate(cls)]
 
     def stats(self) -> dict:

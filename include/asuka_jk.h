@@ -190,7 +190,11 @@ int asuka_jk_stats(asuka_jk_ctx* ctx, asuka_jk_stats_out* out, void* stream);
  * density screening, binned by (A!=B)<<2 | (C!=D)<<1 | (bra pair != ket pair)
  * (the symmetry histogram of asuka/hf/direct_jk_cost.py:134-294), [8] their
  * primitive quartets, [9] quartets evaluated in FP32 (ASUKA_JK_MIXED), [10] their
- * primitive quartets, [11..15] 0. Synchronises the stream. Returns the number of rows. */
+ * primitive quartets, [11] primitive quartets the class-specialised kernels'
+ * loops evaluate after the primitive-quartet cut, [12] the lane slots those
+ * loops take (64 x the longest lane of each wave unit; [11] / [12] is their
+ * SIMD utilisation), [13..15] 0. Synchronises the stream. Returns the number
+ * of rows. */
 int asuka_jk_last_counts(asuka_jk_ctx* ctx, int64_t* rows, int cap, void* stream);
 
 /* Milliseconds between the HIP events recorded (on the build's stream) before
