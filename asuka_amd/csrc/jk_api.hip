@@ -90,6 +90,7 @@ struct ClassPairHost {
   long long nquartets;  // Schwarz-screened quartets
   long long nprim_quartets;
   long long nprim_roots;
+  long long nprim_quartets0;  // primitive quartets before primitive-pair screening (EFM-v1's K)
   double cost;          // estimated work (EFM-v1 FLOPs before density screening), or measured ms once tuned
   int stream;           // 0: the caller's stream, k > 0: the context's side stream k - 1
   int task_units;       // wave units per row task (at most)
@@ -258,30 +259,40 @@ __global__ void k_sym_copy(double* out, const double* X, int n) {
   }
 }
 
-// Routing of a mixed build, one wavefront per wave unit: f32[u] = 1 when the
-// unit has an active quartet and every active quartet's bound Q_ab Q_cd
-// max(D_ab, D_cd) is below tau (the FP32 kernel's vote), f64[u] = 1 when it
-// has an active quartet otherwise; both 0 when density screening removes every
-// quartet of the unit.
-__global__ void k_route_units(DevBasis b, const WaveUnit* __restrict__ units, const int* __restrict__ col_pair,
-                              long long nunits, double eps_density, double tau, int* f64, int* f32) {
-  const int lane = threadIdx.x & 63;
-  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-  for (long long u = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6; u < nunits; u += nw) {
+// Running max of the ket pairs' max|D| within each 64-position block (one
+// wavefront per block): dmax[p] = max over positions block_start..p.
+__global__ void k_pos_dmax(const int* __restrict__ col, const double* __restrict__ Dsp, double* dmax) {
+  const long long p = (long long)blockIdx.x * 64 + threadIdx.x;
+  const int sp = col[p];
+  double v = sp >= 0 ? Dsp[sp] : 0.0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_up(v, off);
+    if ((int)threadIdx.x >= off) v = fmax(v, o);
+  }
+  dmax[p] = v;
+}
+
+// Routing of a mixed build, one thread per wave unit, from an upper bound of
+// its quartets' bounds Q_ab Q_cd max(D_ab, D_cd): (Q_ab Q_first) max(D_ab,
+// max D_cd) with Q_first the unit's largest ket bound (its first position; ket
+// lists are Q-descending) and max D_cd the running block max at its last
+// position. f32[u] = 1 when that bound is below tau (every quartet of the
+// unit is then below tau: the FP32 kernel's share), f64[u] = 1 otherwise;
+// both 0 when it is below eps_density (every quartet density-screened). A
+// unit kept by the bound whose quartets are all screened is skipped in the
+// kernel, which tests each quartet itself.
+__global__ void k_route_units(DevBasis b, const WaveUnit* __restrict__ units, const double* __restrict__ qcol,
+                              const double* __restrict__ dmaxcol, long long nunits, double eps_density, double tau,
+                              int* f64, int* f32) {
+  for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < nunits;
+       u += (long long)gridDim.x * blockDim.x) {
     const WaveUnit un = units[u];
-    bool active = lane < un.ncol;
-    double bound = 0.0;
-    if (active) {
-      const int Qp = col_pair[un.col + lane];
-      bound = (b.Q[un.P] * b.Q[Qp]) * fmax(b.Dsp[un.P], b.Dsp[Qp]);
-      if (eps_density > 0.0) active = bound >= eps_density;
-    }
-    const bool any = __ballot(active) != 0ull;
-    const bool lowp = __ballot(active && !(bound < tau)) == 0ull;
-    if (lane == 0) {
-      f32[u] = (any && lowp) ? 1 : 0;
-      f64[u] = (any && !lowp) ? 1 : 0;
-    }
+    const double hi = (b.Q[un.P] * qcol[un.col]) * fmax(b.Dsp[un.P], dmaxcol[un.col + un.ncol - 1]);
+    const bool any = un.ncol > 0 && (eps_density > 0.0 ? hi >= eps_density : true);
+    const bool lowp = hi < tau;
+    f32[u] = (any && lowp) ? 1 : 0;
+    f64[u] = (any && !lowp) ? 1 : 0;
   }
 }
 
@@ -376,6 +387,9 @@ struct asuka_jk_ctx {
   // remapped row tasks of every class pair (rtask_cap entries each) and the
   // scan scratch
   int *d_rf64 = nullptr, *d_rf32 = nullptr, *d_rp64 = nullptr, *d_rp32 = nullptr;
+  // per ket position: Q of the pair (0 for padding) and, per build, the
+  // running max of the pairs' max|D| from the position's 64-block start
+  double *d_qcol = nullptr, *d_dmaxcol = nullptr;
   WaveUnit *d_units64 = nullptr, *d_units32 = nullptr;
   RowTask *d_rtasks64 = nullptr, *d_rtasks32 = nullptr;
   long long rtask_cap = 0;
@@ -394,6 +408,7 @@ struct asuka_jk_ctx {
   int lao_off[ASUKA_LMAX + 2] = {0};
   size_t lds_per_group = 64 * 1024;  // device limit of LDS per workgroup
   size_t row_acc_max = kRowAccMaxBytes;  // LDS row-accumulator budget (bytes)
+  double eps_prim = kPrimQuartetEps;     // primitive-quartet cut of the register kernels
   int nlao[ASUKA_LMAX + 1] = {0};
   double* d_Q = nullptr;
   double* d_Dsp = nullptr;
@@ -497,6 +512,12 @@ static int pos_layout(asuka_jk_ctx* ctx, const std::vector<int>& col) {
   ctx->pos_len = jblk.back();
   int rc = dev_upload(&ctx->d_jblk, jblk, ctx->owned, nullptr);
   if (rc) return rc;
+  std::vector<double> qcol(col.size(), 0.0);
+  for (size_t t = 0; t < col.size(); ++t)
+    if (col[t] >= 0) qcol[t] = ctx->Q[col[t]];
+  if ((rc = dev_upload(&ctx->d_qcol, qcol, ctx->owned, nullptr))) return rc;
+  HIP_CHECK(hipMalloc((void**)&ctx->d_dmaxcol, sizeof(double) * std::max<size_t>(1, col.size())));
+  ctx->owned.push_back(ctx->d_dmaxcol);
   const size_t bytes = sizeof(double) * (size_t)std::max<long long>(1, 2 * ctx->pos_len);
   HIP_CHECK(hipMalloc((void**)&ctx->d_jpos, bytes));
   ctx->owned.push_back(ctx->d_jpos);
@@ -555,6 +576,10 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
       ctx->lds_per_group = (size_t)v;
+    if (const char* e = std::getenv("ASUKA_EPS_PRIM"); e && *e) {  // measurement of the cut's cost
+      const double v = std::strtod(e, nullptr);
+      if (v >= 0.0 && v <= 1e-10) ctx->eps_prim = v;
+    }
     if (const char* e = std::getenv("ASUKA_ROW_ACC_MAX_KB"); e && *e) {
       const long kb = std::strtol(e, nullptr, 10);
       if (kb >= 0) ctx->row_acc_max = std::min(kRowAccMaxBytes, (size_t)kb * 1024);
@@ -956,6 +981,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     std::vector<int> pairs;  // Q-descending
     std::vector<int> ranks;  // position in the full class list
     std::vector<long long> kpref;  // prefix sums of the pairs' primitive-pair counts
+    std::vector<long long> kpref0;  // the same before primitive-pair screening
     int col_start = 0;
   };
   std::vector<std::vector<PairGroup>> groups((size_t)nmol * ncls);
@@ -979,6 +1005,8 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
         }
       g.kpref.assign(1, 0);
       for (int sp : g.pairs) g.kpref.push_back(g.kpref.back() + ctx->pp_count[sp]);
+      g.kpref0.assign(1, 0);
+      for (int sp : g.pairs) g.kpref0.push_back(g.kpref0.back() + ctx->pp_count0[sp]);
       // every group starts a 64-position block (wave units then start at
       // block boundaries, EriOut::jblk); -1 marks the padding
       col_all.resize((col_all.size() + 63) / 64 * 64, -1);
@@ -1069,7 +1097,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
                     const int kq = kg.pairs[c0 + t];
                     if (ctx->pp_count[kq] > 0) qw = std::max(qw, ctx->ppf[ctx->pp_start[kq]].q);
                   }
-                  const double thr = qw > 0.0 ? kPrimQuartetEps / qw : 1e308;
+                  const double thr = qw > 0.0 ? ctx->eps_prim / qw : 1e308;
                   int k = 0;
                   while (k < nabu && ctx->ppf[ctx->pp_start[P] + k].q >= thr) ++k;
                   nabu = k;
@@ -1097,6 +1125,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
           for (size_t r = 0; r < rowp.size(); ++r) {  // primitive quartets evaluated
             const long long len = (r + 1 < rowp.size() ? offs[r + 1] : off) - offs[r];
             cp.nprim_quartets += (long long)ctx->pp_count[rowp[r]] * (kg.kpref[len] - kg.kpref[0]);
+            cp.nprim_quartets0 += (long long)ctx->pp_count0[rowp[r]] * (kg.kpref0[len] - kg.kpref0[0]);
           }
         }
       }
@@ -1269,7 +1298,7 @@ int asuka_jk_class_stats(const asuka_jk_ctx* ctx, int64_t* rows, int cap) {
     r[4] = cp.nquartets;
     r[5] = cp.nprim_quartets;
     r[6] = cp.nprim_roots;
-    r[7] = 0;
+    r[7] = cp.nprim_quartets0;
     ++n;
   }
   return n;
@@ -1354,9 +1383,11 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
   const long long nu_all = (long long)ctx->h_units.size();
   const bool routed = o.mixed && ctx->d_rf64 && nu_all > 0;
   if (routed) {
-    const long long nb = std::min<long long>((nu_all + 3) / 4, 65535);
-    hipLaunchKernelGGL(k_route_units, dim3((unsigned)nb), dim3(256), 0, s, *dev, ctx->d_units, ctx->d_col_pair,
-                       nu_all, o.eps_density, o.tau_mixed, ctx->d_rf64, ctx->d_rf32);
+    hipLaunchKernelGGL(k_pos_dmax, dim3((unsigned)ctx->nblk), dim3(64), 0, s, ctx->d_col_pair, dev->Dsp,
+                       ctx->d_dmaxcol);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_route_units, dim3(grid_for(nu_all, 256)), dim3(256), 0, s, *dev, ctx->d_units, ctx->d_qcol,
+                       ctx->d_dmaxcol, nu_all, o.eps_density, o.tau_mixed, ctx->d_rf64, ctx->d_rf32);
     HIP_CHECK(hipGetLastError());
     size_t tb = ctx->scan_tmp_bytes;
     HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, tb, ctx->d_rf64, ctx->d_rp64, (int)(nu_all + 1), s));
@@ -1476,7 +1507,7 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
     o.K2 = outs[3];
   }
   o.eps_density = ctx->eps_density;
-  o.eps_prim = ctx->eps_schwarz > 0.0 ? kPrimQuartetEps : 0.0;
+  o.eps_prim = ctx->eps_schwarz > 0.0 ? ctx->eps_prim : 0.0;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
   o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
@@ -1626,7 +1657,7 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   o.D = D;
   o.F = F;
   o.eps_density = ctx->eps_density;
-  o.eps_prim = ctx->eps_schwarz > 0.0 ? kPrimQuartetEps : 0.0;
+  o.eps_prim = ctx->eps_schwarz > 0.0 ? ctx->eps_prim : 0.0;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
   o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
@@ -1732,7 +1763,7 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
   o.J = tjk;
   o.K = tjk + nn;
   o.eps_density = ctx->eps_density;
-  o.eps_prim = ctx->eps_schwarz > 0.0 ? kPrimQuartetEps : 0.0;
+  o.eps_prim = ctx->eps_schwarz > 0.0 ? ctx->eps_prim : 0.0;
   o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
   o.tau_mixed = ctx->tau_mixed;
   // median of reps timed launches (after one untimed) of class pair i as it is now set up

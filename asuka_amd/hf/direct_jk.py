@@ -78,7 +78,8 @@ class DirectJKContext:
     def class_stats(self) -> list[dict]:
         """Per work segment (class pair and ket contraction depth): (la, lb, lc,
         ld), Schwarz-screened shell quartets, primitive quartets and
-        primitive-quartet roots."""
+        primitive-quartet roots, and the primitive quartets counted before
+        primitive-pair screening (prim_quartets0)."""
         info = np.zeros(8, dtype=np.int64)
         _lib.check(_lib.lib().asuka_jk_ctx_info(ctypes.c_void_p(self.handle), info.ctypes.data_as(_lib._i64p)))
         cap = max(1, int(info[5]))
@@ -88,7 +89,7 @@ class DirectJKContext:
         out = []
         for r in rows[:n]:
             out.append({"class": tuple(int(x) for x in r[:4]), "quartets": int(r[4]),
-                        "prim_quartets": int(r[5]), "prim_roots": int(r[6])})
+                        "prim_quartets": int(r[5]), "prim_roots": int(r[6]), "prim_quartets0": int(r[7])})
         return out
 
     def last_counts(self) -> list[dict]:

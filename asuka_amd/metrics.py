@@ -23,6 +23,14 @@ J: nAB (1+[A!=B]) + [bk] nCD (1+[C!=D]); K: (1+[bk]) (nA nC + [C!=D] nA nD +
 
 Inputs are the per-class-pair symmetry histograms the kernels count
 (bin = (A!=B)<<2 | (C!=D)<<1 | bk).
+
+For the nuclear gradient of the two-electron energy, per quartet
+
+  FLOP_grad(q) = K * [24 + n' (24 + 15 (nmax+2)(mmax+2) + 6 N1' + 12 * 4 ncomp)]
+
+with n' = (L+1)//2 + 1 roots and N1' = prod (l+2): the recursions run one
+order higher and each component contracts 12 (centre, xyz) derivative
+products.
 """
 
 from __future__ import annotations
@@ -41,6 +49,16 @@ def flop_eri(la: int, lb: int, lc: int, ld: int, K: int) -> int:
     ncomp = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
     N1 = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1)
     per_root = 24 + 15 * (la + lb + 1) * (lc + ld + 1) + 6 * N1 + 4 * ncomp
+    return int(K) * (24 + n * per_root)
+
+
+def flop_grad(la: int, lb: int, lc: int, ld: int, K: int) -> int:
+    la, lb, lc, ld, K = int(la), int(lb), int(lc), int(ld), int(K)
+    L = la + lb + lc + ld
+    n = (L + 1) // 2 + 1
+    ncomp = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    N1 = (la + 2) * (lb + 2) * (lc + 2) * (ld + 2)
+    per_root = 24 + 15 * (la + lb + 2) * (lc + ld + 2) + 6 * N1 + 12 * 4 * ncomp
     return int(K) * (24 + n * per_root)
 
 
@@ -115,4 +133,4 @@ def roofline_fraction(f64: float, f32: float, seconds: float) -> float:
 
 
 __all__ = ["BuildWork", "PEAK_FP32_TFLOPS", "PEAK_FP64_TFLOPS", "PEAK_HBM_GBS", "build_work", "bytes_alg",
-           "flop_dig", "flop_eri", "roofline_fraction"]
+           "flop_dig", "flop_eri", "flop_grad", "roofline_fraction"]

@@ -40,7 +40,7 @@ METRIC = "Fock-build s & quartets/s: mixed vs FP64 GPU at ≤1e-6 Eh; gradient s
 # by what the counter reads for a 1 GiB device copy), measured with rocprofv3
 # --pmc per workload and mode (profiles/*_hbm_traffic.md); None where not
 # measured.
-HBM_TRAFFIC = {(2, "fp64"): 1.877e10}
+HBM_TRAFFIC = {(2, "fp64"): 1.877e10, (5, "fp64"): 1.1616e12, (5, "mixed"): 8.874e11}
 
 CONFIGS = {
     2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12, kind="jk",
@@ -317,6 +317,66 @@ def run_gradient(args, cfg) -> int:
     g = rhf_nuc_grad(res)
     torch.cuda.synchronize(dev)
     t_full = time.perf_counter() - t0
+
+    # roofline of the derivative kernels (asuka_jk_grad, the step's dominant
+    # launch set): EFM-v1 FLOP_grad over the Schwarz-screened quartets (the
+    # gradient applies no density screening) / their event-timed duration on
+    # the stream they run on
+    from asuka_amd import metrics
+    from asuka_amd.hf.grad import direct_jk_grad
+
+    stats = ctx.class_stats()
+    flop_grad = sum(metrics.flop_grad(*r["class"], r["prim_quartets0"]) for r in stats)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    direct_jk_grad(ctx, D0, 0.5 * D0, -0.25 * D0)
+    torch.cuda.synchronize(dev)
+    reps = max(1, args.steps)
+    e0.record(stream)
+    for _ in range(reps):
+        direct_jk_grad(ctx, D0, 0.5 * D0, -0.25 * D0)
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    t_gk = e0.elapsed_time(e1) / 1e3 / reps
+    achieved = flop_grad / t_gk / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": metrics.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / metrics.PEAK_FP64_TFLOPS, "traffic": None,
+            "flops_grad_efm_v1": flop_grad, "grad_kernel_ms": t_gk * 1e3,
+            "schwarz_quartets": int(sum(r["quartets"] for r in stats))}
+    log(f"derivative kernels: {t_gk * 1e3:.2f} ms, EFM-v1 FLOP_grad {flop_grad / 1e9:.1f} G, {achieved:.2f} TFLOP/s")
+    cpu = None
+    if not args.no_cpu:
+        try:  # the oracle's derivative contraction on a bounded sample of the quartets
+            from oracle import oracle as O
+
+            from asuka_amd.basis import build_ao_basis_cart
+
+            threads = len(os.sched_getaffinity(0))
+            ao, _ = build_ao_basis_cart(mol)
+            octx = O.make_context(ao)
+            Dh = D0.cpu().numpy()
+            Q = ctx.Q_sp
+            ones = np.ones_like(Q)
+            rng = np.random.default_rng(11)
+            ab, cd = sample_screened(Q, ones, cfg["eps"], 0.0, 2000, rng)
+            sa = np.asarray(ao.shell_atom, dtype=np.int32)
+            t1 = time.perf_counter()
+            O.grad2e_tasks(octx, Dh, 0.5 * Dh, -0.25 * Dh, sa, mol.natm, ab, cd, hp=False, threads=threads)
+            rate0 = ab.size / (time.perf_counter() - t1)
+            nq = roof["schwarz_quartets"]
+            nsamp = int(min(nq, max(2000, rate0 * args.cpu_seconds)))
+            ab, cd = sample_screened(Q, ones, cfg["eps"], 0.0, nsamp, rng)
+            t1 = time.perf_counter()
+            O.grad2e_tasks(octx, Dh, 0.5 * Dh, -0.25 * Dh, sa, mol.natm, ab, cd, hp=False, threads=threads)
+            tc = time.perf_counter() - t1
+            rate = ab.size / tc
+            cpu = {"value": rate / nq, "unit": "gradient_evals/s (derivative contraction, extrapolated)",
+                   "cores": threads, "kind": "port",
+                   "sample": f"{ab.size} of the {nq} Schwarz-screened quartets drawn uniformly, the oracle's "
+                             f"adjoint tile and derivative contraction per quartet, {tc:.1f} s on {cpu_model()}",
+                   "quartets_per_s": rate}
+        except Exception as e:  # the baseline is reported, never required
+            log("oracle leg skipped:", repr(e))
     fd = None
     if args.fd:
         # central differences (step 1e-4 Bohr) of the converged SCF energy on 6
@@ -368,6 +428,8 @@ def run_gradient(args, cfg) -> int:
         "e_tot": res.scf.e_tot,
         "grad_2e_norm": float(g2.norm()),
         "fd_check": fd,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -584,8 +646,11 @@ def main() -> int:
         parity = {"sampled_tiles": int(sab.size), "max_abs_dtile_vs_oracle": dmax}
         if not args.no_cpu:
             threads = len(os.sched_getaffinity(0))
-            # calibrate on a small sample, then size the sample to ~cpu_seconds
-            cab, ccd = sample_screened(Q, Dsp, cfg["eps"], ctx.eps_density, 20000, rng)
+            # calibrate on a small sample (after a warm-up call: thread pool,
+            # page faults), then size the sample to ~cpu_seconds
+            cab, ccd = sample_screened(Q, Dsp, cfg["eps"], ctx.eps_density, 2000, rng)
+            O.direct_jk_tasks(octx, D_h, cab, ccd, hp=False, threads=threads)
+            cab, ccd = sample_screened(Q, Dsp, cfg["eps"], ctx.eps_density, 100000, rng)
             t1 = time.perf_counter()
             O.direct_jk_tasks(octx, D_h, cab, ccd, hp=False, threads=threads)
             rate0 = cab.size / (time.perf_counter() - t1)

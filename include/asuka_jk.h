@@ -156,8 +156,9 @@ int asuka_jk_shell_pairs(const asuka_jk_ctx* ctx, int32_t* sp_A, int32_t* sp_B, 
 
 /* Quartet statistics per class pair, for the throughput metrics. Fills up to
  * `cap` rows of 8 int64: la, lb, lc, ld, schwarz-screened quartets, primitive
- * quartets, sum over quartets of primitive-quartets x roots, 0. Returns the number
- * of rows (>= 0) or an error code. */
+ * quartets, sum over quartets of primitive-quartets x roots, primitive quartets
+ * counted before primitive-pair screening. Returns the number of rows (>= 0)
+ * or an error code. */
 int asuka_jk_class_stats(const asuka_jk_ctx* ctx, int64_t* rows, int cap);
 
 /* Work summary of a context (asuka_jk_stats): the Schwarz-screened quartet

@@ -840,3 +840,39 @@ void oracle_grad2e(const oracle_basis* bs, int nao, const double* DwJ, const dou
     free(gp);
   }
 }
+
+/* The same contraction over a list of (spAB, spCD) tasks only (a bounded
+ * sample of the quartets, for timing): grad (natm, 3) is overwritten with
+ * the sum of the tasks' derivative contributions. */
+void oracle_grad2e_tasks(const oracle_basis* bs, int nao, const double* DwJ, const double* DwK, const double* Dc,
+                         const int32_t* shell_atom, int natm, int hp, int nthreads, long ntask,
+                         const int32_t* task_ab, const int32_t* task_cd, double* grad) {
+  memset(grad, 0, sizeof(double) * 3 * natm);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    double* bar = (double*)malloc(sizeof(double) * NCART_MAX * NCART_MAX * NCART_MAX * NCART_MAX);
+    double* gp = (double*)calloc(3 * (size_t)natm, sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+    for (long t = 0; t < ntask; ++t) {
+      const int ab = task_ab[t], cd = task_cd[t];
+      double out[12] = {0};
+      bar_tile(bs, ab, cd, nao, DwJ, DwK, Dc, bar);
+      deriv_contract_sp(bs, ab, cd, hp, bar, out);
+      int at[4] = {shell_atom[bs->sp_A[ab]], shell_atom[bs->sp_B[ab]], shell_atom[bs->sp_A[cd]],
+                   shell_atom[bs->sp_B[cd]]};
+      for (int c = 0; c < 4; ++c)
+        for (int ax = 0; ax < 3; ++ax) gp[3 * at[c] + ax] += out[3 * c + ax];
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (int i = 0; i < 3 * natm; ++i) grad[i] += gp[i];
+    free(bar);
+    free(gp);
+  }
+}

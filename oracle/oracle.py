@@ -80,6 +80,9 @@ def lib():
                                            ctypes.c_double, _i32p, _i32p, _i64p, ctypes.c_int, ctypes.c_int, _i64p]
         L.oracle_grad2e.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, _f64p, _f64p, _f64p, _i32p,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p]
+        L.oracle_grad2e_tasks.argtypes = [ctypes.POINTER(_OracleBasis), ctypes.c_int, _f64p, _f64p, _f64p, _i32p,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long, _i32p, _i32p,
+                                          _f64p]
         _LIB = L
     return _LIB
 
@@ -478,6 +481,24 @@ def grad2e(ctx: OracleContext, D: np.ndarray, WJ: np.ndarray | None, WK: np.ndar
     g = np.zeros((int(natm), 3))
     lib().oracle_grad2e(ctypes.byref(ctx.struct), ctx.nao, _p(wj, _f64p), _p(wk, _f64p), _p(D, _f64p),
                         _p(sa, _i32p), int(natm), int(bool(hp)), int(threads), _p(g, _f64p))
+    return g
+
+
+def grad2e_tasks(ctx: OracleContext, D: np.ndarray, WJ: np.ndarray | None, WK: np.ndarray | None,
+                 shell_atom: np.ndarray, natm: int, ab: np.ndarray, cd: np.ndarray, hp: bool = True,
+                 threads: int = 0) -> np.ndarray:
+    """:func:`grad2e`'s per-quartet contraction over the given (spAB, spCD)
+    tasks only (a bounded sample, for timing the CPU baseline)."""
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    wj = None if WJ is None else np.ascontiguousarray(WJ, dtype=np.float64)
+    wk = None if WK is None else np.ascontiguousarray(-2.0 * np.asarray(WK), dtype=np.float64)
+    sa = np.ascontiguousarray(shell_atom, dtype=np.int32)
+    ab = np.ascontiguousarray(ab, dtype=np.int32)
+    cd = np.ascontiguousarray(cd, dtype=np.int32)
+    g = np.zeros((int(natm), 3))
+    lib().oracle_grad2e_tasks(ctypes.byref(ctx.struct), ctx.nao, _p(wj, _f64p), _p(wk, _f64p), _p(D, _f64p),
+                              _p(sa, _i32p), int(natm), int(bool(hp)), int(threads), ctypes.c_long(ab.size),
+                              _p(ab, _i32p), _p(cd, _i32p), _p(g, _f64p))
     return g
 
 

@@ -42,3 +42,14 @@ def test_build_work_sums():
     assert w.prim_quartets == 10
     assert w.flop_eri == 10 * 73
     assert w.flop_dig == 1 * 4 + 2 * 12
+
+
+def test_flop_grad_closed_forms():
+    from asuka_amd import metrics
+
+    # (ss|ss): one root, (nmax+2)(mmax+2) = 4, N1' = 16, one component
+    assert metrics.flop_grad(0, 0, 0, 0, 1) == 24 + (24 + 15 * 4 + 6 * 16 + 48)
+    # (pp|pp): n' = 3, 16 VRR entries, N1' = 81, 81 components, K = 3
+    assert metrics.flop_grad(1, 1, 1, 1, 3) == 3 * (24 + 3 * (24 + 15 * 16 + 6 * 81 + 48 * 81))
+    # linear in K
+    assert metrics.flop_grad(2, 1, 1, 0, 7) == 7 * metrics.flop_grad(2, 1, 1, 0, 1)

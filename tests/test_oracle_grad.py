@@ -109,3 +109,30 @@ def test_grad2e_bilinear(oracle):
     gJ = oracle.grad2e(ctx, D, 0.5 * D, None, ao.shell_atom, mol.natm, hp=False, threads=0)
     gK = oracle.grad2e(ctx, D, None, -0.25 * D, ao.shell_atom, mol.natm, hp=False, threads=0)
     assert np.abs(gJ + gK - g1).max() < 1e-10 * np.abs(g1).max()
+
+
+def test_grad2e_tasks_over_all_ordered_pairs_equals_grad2e(oracle):
+    """The task-list driver (the gradient CPU baseline's) sums the same per-quartet
+    contributions as the full ordered-pair driver."""
+    import numpy as np
+
+    from asuka_amd.basis import build_ao_basis_cart
+    from asuka_amd.molecule import water_monomer
+
+    mol = water_monomer("cc-pVDZ")
+    ao, _ = build_ao_basis_cart(mol)
+    octx = oracle.make_context(ao)
+    C = np.random.default_rng(5).standard_normal((ao.nao, 5))
+    D = C @ C.T
+    nsp = len(oracle.shell_pairs_l_order(ao)[0])
+    ab = np.repeat(np.arange(nsp), nsp)
+    cd = np.tile(np.arange(nsp), nsp)
+    sa = np.asarray(ao.shell_atom, dtype=np.int32)
+    g1 = oracle.grad2e(octx, D, 0.5 * D, -0.25 * D, sa, mol.natm, hp=False)
+    g2 = oracle.grad2e_tasks(octx, D, 0.5 * D, -0.25 * D, sa, mol.natm, ab, cd, hp=False)
+    assert np.abs(g1 - g2).max() < 1e-11 * max(1.0, np.abs(g1).max())
+    # a subset: half the tasks plus the other half equals the whole
+    h = ab.size // 2
+    g3 = oracle.grad2e_tasks(octx, D, 0.5 * D, -0.25 * D, sa, mol.natm, ab[:h], cd[:h], hp=False)
+    g4 = oracle.grad2e_tasks(octx, D, 0.5 * D, -0.25 * D, sa, mol.natm, ab[h:], cd[h:], hp=False)
+    assert np.abs(g3 + g4 - g2).max() < 1e-11 * max(1.0, np.abs(g1).max())
