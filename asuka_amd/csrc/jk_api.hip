@@ -161,7 +161,8 @@ __global__ void k_symmetrize(double* X, int n) {
 
 // max |D| over each shell-pair block, both orientations (asuka/hf/direct_jk.py:1484-1527);
 // with a second density D2 (two-density builds) the max over both.
-__global__ void k_dsp(const double* D, const double* D2, DevBasis b, double* Dsp) {
+__global__ void k_dsp(const double* D, const double* D2, DevBasis b, double* Dsp, double* kd,
+                      const int* __restrict__ pos_of_sp) {
   const int sp = blockIdx.x * blockDim.x + threadIdx.x;
   if (sp >= b.nsp) return;
   const int A = b.sp_A[sp], B = b.sp_B[sp];
@@ -182,6 +183,7 @@ __global__ void k_dsp(const double* D, const double* D2, DevBasis b, double* Dsp
       }
     }
   Dsp[sp] = m;
+  if (kd && pos_of_sp[sp] >= 0) kd[pos_of_sp[sp]] = m;  // the same value by ket position
 }
 
 // Ket-position buffers (EriOut::jblk), one 64-thread workgroup per block:
@@ -390,6 +392,9 @@ struct asuka_jk_ctx {
   // per ket position: Q of the pair (0 for padding) and, per build, the
   // running max of the pairs' max|D| from the position's 64-block start
   double *d_qcol = nullptr, *d_dmaxcol = nullptr;
+  double* d_kd = nullptr;          // DevBasis::kd
+  int* d_pos_of_sp = nullptr;      // position of each shell pair in the ket lists (-1: none)
+  std::vector<int> h_shell_lao;    // host copy of DevBasis::shell_lao
   WaveUnit *d_units64 = nullptr, *d_units32 = nullptr;
   RowTask *d_rtasks64 = nullptr, *d_rtasks32 = nullptr;
   long long rtask_cap = 0;
@@ -512,10 +517,36 @@ static int pos_layout(asuka_jk_ctx* ctx, const std::vector<int>& col) {
   ctx->pos_len = jblk.back();
   int rc = dev_upload(&ctx->d_jblk, jblk, ctx->owned, nullptr);
   if (rc) return rc;
-  std::vector<double> qcol(col.size(), 0.0);
-  for (size_t t = 0; t < col.size(); ++t)
-    if (col[t] >= 0) qcol[t] = ctx->Q[col[t]];
+  std::vector<double> qcol(col.size(), 0.0), kcd(3 * col.size(), 0.0);
+  std::vector<int2> kpp(col.size(), int2{0, 0}), kao(col.size(), int2{0, 0}), klao(col.size(), int2{0, 0});
+  std::vector<int> pos_of_sp(ctx->nsp, -1);
+  for (size_t t = 0; t < col.size(); ++t) {
+    const int sp = col[t];
+    if (sp < 0) continue;
+    const int C = ctx->sp_A[sp], Dh = ctx->sp_B[sp];
+    qcol[t] = ctx->Q[sp];
+    kpp[t] = int2{ctx->pp_start[sp], ctx->pp_count[sp]};
+    kao[t] = int2{ctx->shell_ao[C], ctx->shell_ao[Dh]};
+    klao[t] = int2{ctx->h_shell_lao[C], ctx->h_shell_lao[Dh]};
+    for (int x = 0; x < 3; ++x) kcd[3 * t + x] = ctx->shell_xyz[3 * C + x] - ctx->shell_xyz[3 * Dh + x];
+    pos_of_sp[sp] = (int)t;
+  }
   if ((rc = dev_upload(&ctx->d_qcol, qcol, ctx->owned, nullptr))) return rc;
+  int2 *d_kpp, *d_kao, *d_klao;
+  double* d_kcd;
+  if ((rc = dev_upload(&d_kpp, kpp, ctx->owned, nullptr))) return rc;
+  if ((rc = dev_upload(&d_kao, kao, ctx->owned, nullptr))) return rc;
+  if ((rc = dev_upload(&d_klao, klao, ctx->owned, nullptr))) return rc;
+  if ((rc = dev_upload(&d_kcd, kcd, ctx->owned, nullptr))) return rc;
+  if ((rc = dev_upload(&ctx->d_pos_of_sp, pos_of_sp, ctx->owned, nullptr))) return rc;
+  std::vector<double> kd0(col.size(), 0.0);  // padding positions stay 0
+  if ((rc = dev_upload(&ctx->d_kd, kd0, ctx->owned, nullptr))) return rc;
+  ctx->dev.kq = ctx->d_qcol;
+  ctx->dev.kd = ctx->d_kd;
+  ctx->dev.kpp = d_kpp;
+  ctx->dev.kao = d_kao;
+  ctx->dev.klao = d_klao;
+  ctx->dev.kcd = d_kcd;
   HIP_CHECK(hipMalloc((void**)&ctx->d_dmaxcol, sizeof(double) * std::max<size_t>(1, col.size())));
   ctx->owned.push_back(ctx->d_dmaxcol);
   const size_t bytes = sizeof(double) * (size_t)std::max<long long>(1, 2 * ctx->pos_len);
@@ -737,6 +768,7 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
     if ((rc = dev_upload(&d_mlao, mol_lao, own, s))) return rc;
     if ((rc = dev_upload(&d_smol, ctx->shell_mol, own, s))) return rc;
     ctx->dev.shell_lao = d_slao;
+    ctx->h_shell_lao = shell_lao;
     ctx->dev.lao_ao = d_laoao;
     ctx->dev.mol_lao = d_mlao;
     ctx->dev.shell_mol = d_smol;
@@ -1494,7 +1526,8 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
     if (X) hipLaunchKernelGGL(k_fill, dim3(grid_for(nn, 256)), dim3(256), 0, s, X, nn, 0.0);
   DevBasis dev = ctx->dev;
   if (dsp) dev.Dsp = dsp;
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, D2, dev, (double*)dev.Dsp);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, D2, dev, (double*)dev.Dsp, ctx->d_kd,
+                     ctx->d_pos_of_sp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_JK;
@@ -1650,7 +1683,8 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   hipStream_t s = (hipStream_t)stream;
   const long long nn = (long long)ctx->nao * ctx->nao;
   if (F != hcore) hipLaunchKernelGGL(k_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, F, hcore, nn);
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev, ctx->d_Dsp);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev, ctx->d_Dsp,
+                     ctx->d_kd, ctx->d_pos_of_sp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_FOCK;
@@ -1755,7 +1789,8 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
       (void)hipEventDestroy(b);
     }
   } free_ev{e0, e1};
-  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev, ctx->d_Dsp);
+  hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev, ctx->d_Dsp,
+                     ctx->d_kd, ctx->d_pos_of_sp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_JK;

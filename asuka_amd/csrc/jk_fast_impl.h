@@ -235,8 +235,9 @@ struct RowAcc {
 // or -1 for D[c,d] from D and J[c,d] atomics on J.
 
 template <class PL, int CH, typename CT>
-__device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
-                                          const CT* __restrict__ tab, const RowAcc& acc, double* jcd, long long jb) {
+__device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, int pos, int nab,
+                                          bool active, const CT* __restrict__ tab, const RowAcc& acc, double* jcd,
+                                          long long jb) {
   using CK = Chunk<PL, CH>;
   constexpr int NEL = CK::NEL, NFL = CK::NFL, NR = PL::NR;
   constexpr int NMAX = PL::NMAX, MMAX = PL::MMAX;
@@ -252,11 +253,12 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 
   // bra-side indices are wave-uniform; readfirstlane states it so that the
   // loads addressed by them are scalar
-  const int A = uni(b.sp_A[P]), B = uni(b.sp_B[P]), C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
+  // the ket pair's data by its position (pos: contiguous across the lanes)
+  const int A = uni(b.sp_A[P]), B = uni(b.sp_B[P]);
   const double Ax = b.shell_xyz[3 * A], Ay = b.shell_xyz[3 * A + 1], Az = b.shell_xyz[3 * A + 2];
-  const double Cx = b.shell_xyz[3 * C], Cy = b.shell_xyz[3 * C + 1], Cz = b.shell_xyz[3 * C + 2];
   const int ab0 = uni(b.pp_start[P]);
-  const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
+  const int2 kpp = b.kpp[pos];
+  const int cd0 = kpp.x, ncd = active ? kpp.y : 0;
   // the primitive pairs in the precision of CT (the FP32 path reads the float
   // copy: no conversions in the primitive loop but P - Q, formed in FP64)
   using PPT = typename std::conditional<sizeof(CT) == 4, PrimPairF32, PrimPairF>::type;
@@ -343,11 +345,10 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 
   // ---- horizontal transfer: ket (C-components of the chunk) then bra ----
   const double Bx = b.shell_xyz[3 * B], By = b.shell_xyz[3 * B + 1], Bz = b.shell_xyz[3 * B + 2];
-  const double Dx = b.shell_xyz[3 * Dsh], Dy = b.shell_xyz[3 * Dsh + 1], Dz = b.shell_xyz[3 * Dsh + 2];
   CT pab[3][LB + 1], pcd[3][LD + 1];
   {
     const CT ab[3] = {CT(Ax - Bx), CT(Ay - By), CT(Az - Bz)};
-    const CT cd[3] = {CT(Cx - Dx), CT(Cy - Dy), CT(Cz - Dz)};
+    const CT cd[3] = {CT(b.kcd[3 * pos]), CT(b.kcd[3 * pos + 1]), CT(b.kcd[3 * pos + 2])};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
       pab[ax][0] = one;
@@ -412,14 +413,16 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   // (two-density builds) the same tile is digested again into (J2, K2).
   long long N, msh;  // the molecule's D/J/K block (uniform: the bra pair is)
   mol_layout(b, A, N, msh);
-  const int aoA = b.shell_ao[A] + a0, aoB = b.shell_ao[B], aoC = b.shell_ao[C] + c0, aoD = b.shell_ao[Dsh];
-  const bool ab_neq = A != B, cd_neq = C != Dsh, bk = P != Qp;
+  const int2 kao = b.kao[pos];
+  const int aoA = b.shell_ao[A] + a0, aoB = b.shell_ao[B], aoC = kao.x + c0, aoD = kao.y;
+  const bool ab_neq = A != B, cd_neq = kao.x != kao.y, bk = P != Qp;
   const double fj = (ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0);
   const double fk = bk ? 2.0 : 1.0;
   const double ks = (o.mode == MODE_FOCK) ? -0.5 * fk : fk;
   const bool lds = acc.on;  // uniform
   // compact columns of the lane's C (first component of the chunk) and D shells
-  const int colC = lds ? b.shell_lao[C] + c0 : 0, colD = lds ? b.shell_lao[Dsh] : 0;
+  const int2 klao = b.klao[pos];
+  const int colC = lds ? klao.x + c0 : 0, colD = lds ? klao.y : 0;
   const int nden = o.D2 ? 2 : 1;  // uniform
   for (int di = 0; di < nden; ++di) {
     const double* Dm = (di ? o.D2 : o.D) + msh;
@@ -567,26 +570,26 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 }
 
 template <class PL, int CH, typename CT>
-__device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp, int nab,
-                                               bool active, const CT* __restrict__ tab, const RowAcc& acc,
+__device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp, int pos,
+                                               int nab, bool active, const CT* __restrict__ tab, const RowAcc& acc,
                                                long long jb) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH, CT>(b, o, P, Qp, nab, active, tab, acc, nullptr, jb);
+      run_chunk<PL, CH, CT>(b, o, P, Qp, pos, nab, active, tab, acc, nullptr, jb);
       return;
     }
-    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, nab, active, tab, acc, jb);
+    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, pos, nab, active, tab, acc, jb);
   }
 }
 
 // The A chunks CA.. of C chunk CC, in sequence, carrying J[c,d] in jcd.
 template <class PL, int CC, int CA, typename CT>
-__device__ __forceinline__ void run_agroup(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
-                                           const CT* __restrict__ tab, const RowAcc& acc, double* jcd,
+__device__ __forceinline__ void run_agroup(const DevBasis& b, const EriOut& o, int P, int Qp, int pos, int nab,
+                                           bool active, const CT* __restrict__ tab, const RowAcc& acc, double* jcd,
                                            long long jb) {
   if constexpr (CA < PL::SPLIT_A) {
-    run_chunk<PL, CA * PL::SPLIT_C + CC, CT>(b, o, P, Qp, nab, active, tab, acc, jcd, jb);
-    run_agroup<PL, CC, CA + 1, CT>(b, o, P, Qp, nab, active, tab, acc, jcd, jb);
+    run_chunk<PL, CA * PL::SPLIT_C + CC, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
+    run_agroup<PL, CC, CA + 1, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
   }
 }
 
@@ -594,14 +597,14 @@ __device__ __forceinline__ void run_agroup(const DevBasis& b, const EriOut& o, i
 // with the J[c,d] sums carried across the A chunks in registers, so J[c,d]
 // receives one atomic per element and quartet instead of one per A chunk.
 template <class PL, int CC, typename CT>
-__device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, int P, int Qp, int nab, bool active,
-                                          const CT* __restrict__ tab, const RowAcc& acc, long long jb) {
+__device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, int P, int Qp, int pos, int nab,
+                                          bool active, const CT* __restrict__ tab, const RowAcc& acc, long long jb) {
   using CK = Chunk<PL, CC>;  // A chunk 0: the C range of every A chunk of CC
   constexpr int ND = PL::ND, NCDL = (CK::c1 - CK::c0) * ND;
   double jcd[NCDL];
 #pragma unroll
   for (int c = 0; c < NCDL; ++c) jcd[c] = 0.0;
-  run_agroup<PL, CC, 0, CT>(b, o, P, Qp, nab, active, tab, acc, jcd, jb);
+  run_agroup<PL, CC, 0, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
   double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
   if (Jo && active && P != Qp && jb >= 0) {
     const int lane = threadIdx.x & 63;
@@ -621,17 +624,17 @@ __device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, in
 // Wave item `item` of a unit: a C chunk with its A chunks for CARRY plans,
 // one chunk otherwise.
 template <class PL, int CC, typename CT>
-__device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const EriOut& o, int P, int Qp, int nab,
-                                               bool active, const CT* __restrict__ tab, const RowAcc& acc,
+__device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const EriOut& o, int P, int Qp, int pos,
+                                               int nab, bool active, const CT* __restrict__ tab, const RowAcc& acc,
                                                long long jb) {
   if constexpr (!PL::CARRY) {
-    dispatch_chunk<PL, 0, CT>(cc, b, o, P, Qp, nab, active, tab, acc, jb);
+    dispatch_chunk<PL, 0, CT>(cc, b, o, P, Qp, pos, nab, active, tab, acc, jb);
   } else if constexpr (CC < PL::SPLIT_C) {
     if (cc == CC) {
-      run_citem<PL, CC, CT>(b, o, P, Qp, nab, active, tab, acc, jb);
+      run_citem<PL, CC, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jb);
       return;
     }
-    dispatch_citem<PL, CC + 1, CT>(cc, b, o, P, Qp, nab, active, tab, acc, jb);
+    dispatch_citem<PL, CC + 1, CT>(cc, b, o, P, Qp, pos, nab, active, tab, acc, jb);
   }
 }
 
@@ -750,10 +753,13 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       const int ui = item - chunk * tk.nu;
       const WaveUnit un = w.units[tk.u0 + ui];
       bool active = lane < un.ncol;
-      const int Qp = w.col_pair[un.col + (active ? lane : 0)];
+      // the lane's ket position: units start at 64-position block boundaries,
+      // so every lane's position (tail lanes: padding) is inside the lists
+      const int pos = un.col + lane;
+      const int Qp = w.col_pair[active ? pos : un.col];
       double bound = 0.0;
       if (active) {
-        bound = (b.Q[P] * b.Q[Qp]) * fmax(b.Dsp[P], b.Dsp[Qp]);
+        bound = (b.Q[P] * b.kq[pos]) * fmax(b.Dsp[P], b.kd[pos]);
         if (o.eps_density > 0.0) active = bound >= o.eps_density;
       }
       if (!wave_any(active)) continue;  // the whole unit is density-screened
@@ -806,7 +812,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       const int nab = uni(o.eps_prim > 0.0 ? un.nab : b.pp_count[P]);
       // the unit's block of the ket-position buffers (units start at block boundaries)
       const long long jb = o.jblk ? o.jblk[un.col >> 6] : -1LL;
-      dispatch_citem<PL, 0, CT>(chunk, b, o, P, Qp, nab, active, tab, acc, jb);
+      dispatch_citem<PL, 0, CT>(chunk, b, o, P, Qp, pos, nab, active, tab, acc, jb);
     }
     if (use_acc) {
       if (worked && lane == 0) s_work = 1;

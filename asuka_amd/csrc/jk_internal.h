@@ -81,6 +81,17 @@ struct DevBasis {
   const long long* mol_shift;  // (nmol) element offset of that block (see mol_layout)
   const double* Q;           // (nsp) Schwarz bounds
   const double* Dsp;         // (nsp) max |D| per shell-pair block
+  // Per ket position (the padded ket lists of the wave units, EriOut::jblk),
+  // read with contiguous per-lane loads instead of gathers through the pair
+  // index: the pair's Q and max |D| (kd, written by each build's k_dsp), its
+  // first primitive pair and count, the first AOs of its shells C and D,
+  // their compact (per-l) columns, and C - D (3 doubles per position).
+  const double* kq;
+  const double* kd;
+  const int2* kpp;
+  const int2* kao;
+  const int2* klao;
+  const double* kcd;
   int nao;
   int nsp;
 };
