@@ -541,6 +541,51 @@ static int pos_layout(asuka_jk_ctx* ctx, const std::vector<int>& col) {
   if ((rc = dev_upload(&ctx->d_pos_of_sp, pos_of_sp, ctx->owned, nullptr))) return rc;
   std::vector<double> kd0(col.size(), 0.0);  // padding positions stay 0
   if ((rc = dev_upload(&ctx->d_kd, kd0, ctx->owned, nullptr))) return rc;
+  {  // transposed ket primitive pairs (DevBasis::kpoff)
+    const long long nb = (long long)col.size() / 64;
+    std::vector<long long> kpoff((size_t)nb + 1, 0);
+    for (long long k = 0; k < nb; ++k) {
+      int kmax = 0;
+      for (int t = 0; t < 64; ++t) {
+        const int sp = col[(size_t)(k * 64 + t)];
+        if (sp >= 0) kmax = std::max(kmax, ctx->pp_count[sp]);
+      }
+      kpoff[(size_t)k + 1] = kpoff[(size_t)k] + kmax;
+    }
+    const long long nslot = std::max<long long>(1, kpoff.back() * 64);
+    std::vector<double> k64((size_t)(10 * nslot), 0.0), k32d((size_t)(3 * nslot), 0.0);
+    std::vector<float> k32f((size_t)(7 * nslot), 0.0f);
+    for (long long k = 0; k < nb; ++k)
+      for (int t = 0; t < 64; ++t) {
+        const int sp = col[(size_t)(k * 64 + t)];
+        if (sp < 0) continue;
+        for (int j = 0; j < ctx->pp_count[sp]; ++j) {
+          const PrimPairF& f = ctx->ppf[(size_t)ctx->pp_start[sp] + j];
+          const size_t slot = (size_t)((kpoff[(size_t)k] + j) * 64 + t), n = (size_t)nslot;
+          const double v64[10] = {f.eta, f.inv_eta, f.ck, f.Px, f.Py, f.Pz, f.PAx, f.PAy, f.PAz, f.q};
+          for (int x = 0; x < 10; ++x) k64[x * n + slot] = v64[x];
+          k32d[0 * n + slot] = f.Px;
+          k32d[1 * n + slot] = f.Py;
+          k32d[2 * n + slot] = f.Pz;
+          // the same roundings as the FP32 pair table (q clamped to the float range)
+          const float v32[7] = {(float)f.eta, (float)f.inv_eta, (float)f.ck, (float)f.PAx, (float)f.PAy, (float)f.PAz,
+                                (float)std::min(f.q, 3e38)};
+          for (int x = 0; x < 7; ++x) k32f[x * n + slot] = v32[x];
+        }
+      }
+    long long* d_kpoff;
+    double *d_k64, *d_k32d;
+    float* d_k32f;
+    if ((rc = dev_upload(&d_kpoff, kpoff, ctx->owned, nullptr))) return rc;
+    if ((rc = dev_upload(&d_k64, k64, ctx->owned, nullptr))) return rc;
+    if ((rc = dev_upload(&d_k32d, k32d, ctx->owned, nullptr))) return rc;
+    if ((rc = dev_upload(&d_k32f, k32f, ctx->owned, nullptr))) return rc;
+    ctx->dev.kpoff = d_kpoff;
+    ctx->dev.kp64 = d_k64;
+    ctx->dev.kp32d = d_k32d;
+    ctx->dev.kp32f = d_k32f;
+    ctx->dev.kslots = nslot;
+  }
   ctx->dev.kq = ctx->d_qcol;
   ctx->dev.kd = ctx->d_kd;
   ctx->dev.kpp = d_kpp;

@@ -191,6 +191,46 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 
+// The ket primitive pair in slot t of the transposed tables (DevBasis::kpoff).
+template <typename PPT>
+__device__ __forceinline__ PPT load_ket(const DevBasis& b, long long t);
+template <>
+__device__ __forceinline__ PrimPairF load_ket<PrimPairF>(const DevBasis& b, long long t) {
+  const double* k = b.kp64 + t;
+  const long long n = b.kslots;
+  PrimPairF y;
+  y.eta = k[0];
+  y.inv_eta = k[n];
+  y.ck = k[2 * n];
+  y.Px = k[3 * n];
+  y.Py = k[4 * n];
+  y.Pz = k[5 * n];
+  y.PAx = k[6 * n];
+  y.PAy = k[7 * n];
+  y.PAz = k[8 * n];
+  y.q = k[9 * n];
+  return y;
+}
+template <>
+__device__ __forceinline__ PrimPairF32 load_ket<PrimPairF32>(const DevBasis& b, long long t) {
+  const double* d = b.kp32d + t;
+  const float* f = b.kp32f + t;
+  const long long n = b.kslots;
+  PrimPairF32 y;
+  y.Px = d[0];
+  y.Py = d[n];
+  y.Pz = d[2 * n];
+  y.eta = f[0];
+  y.inv_eta = f[n];
+  y.ck = f[2 * n];
+  y.PAx = f[3 * n];
+  y.PAy = f[4 * n];
+  y.PAz = f[5 * n];
+  y.q = f[6 * n];
+  y.pad = 0.0f;
+  return y;
+}
+
 // Arithmetic helpers for the compute type CT of the recursion (double, or
 // float on the FP32 path).
 __device__ __forceinline__ double fmac(double a, double b, double c) { return fma(a, b, c); }
@@ -258,7 +298,7 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   const double Ax = b.shell_xyz[3 * A], Ay = b.shell_xyz[3 * A + 1], Az = b.shell_xyz[3 * A + 2];
   const int ab0 = uni(b.pp_start[P]);
   const int2 kpp = b.kpp[pos];
-  const int cd0 = kpp.x, ncd = active ? kpp.y : 0;
+  const int ncd = active ? kpp.y : 0;  // (kpp.x, the first pair in the pair table, is for host-side use)
   // the primitive pairs in the precision of CT (the FP32 path reads the float
   // copy: no conversions in the primitive loop but P - Q, formed in FP64)
   using PPT = typename std::conditional<sizeof(CT) == 4, PrimPairF32, PrimPairF>::type;
@@ -279,12 +319,15 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   // at the first ket pair whose q times the largest bra bound is below it.
   // Every skipped primitive quartet adds less than eps_prim to each integral.
   const CT kcut = (nab > 0 && o.eps_prim > 0.0) ? CT(o.eps_prim / (double)ppf[ab0].q) : CT(0);
+  // the lane's ket primitive pairs, transposed by position (the lanes' j-th
+  // pairs are contiguous): slots ks0 + 64 j
+  const long long ks0 = b.kpoff[uni(pos >> 6)] * 64 + (pos & 63);
   PPT Yn;
-  if (ncd > 0) Yn = ppf[cd0];
+  if (ncd > 0) Yn = load_ket<PPT>(b, ks0);
   for (int j = 0; j < ncd; ++j) {
     const PPT Y = Yn;
     if (Y.q < kcut) break;  // per lane: the rest of its ket pairs are smaller
-    if (j + 1 < ncd) Yn = ppf[cd0 + j + 1];  // next ket pair, in flight during this one
+    if (j + 1 < ncd) Yn = load_ket<PPT>(b, ks0 + 64LL * (j + 1));  // next ket pair, in flight during this one
     const CT q = CT(Y.eta), iq = CT(Y.inv_eta), ckq = CT(Y.ck);
     const CT QCx = CT(Y.PAx), QCy = CT(Y.PAy), QCz = CT(Y.PAz);
     for (int i = 0; i < nab; ++i) {

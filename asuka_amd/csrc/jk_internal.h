@@ -92,6 +92,17 @@ struct DevBasis {
   const int2* kao;
   const int2* klao;
   const double* kcd;
+  // The ket primitive pairs by position, transposed so that the lanes of a
+  // wave read the j-th primitive pair of their kets at consecutive addresses:
+  // block k's primitive j of lane l is slot (kpoff[k] + j) * 64 + l, field f
+  // of slot t at kp64[f * kslots + t] (PrimPairF's ten fields in order; FP64
+  // kernels) and, for the FP32 kernels, kp32d[f * kslots + t] (Px, Py, Pz)
+  // and kp32f[f * kslots + t] (eta, inv_eta, ck, PAx, PAy, PAz, q).
+  const long long* kpoff;
+  const double* kp64;
+  const double* kp32d;
+  const float* kp32f;
+  long long kslots;
   int nao;
   int nsp;
 };
