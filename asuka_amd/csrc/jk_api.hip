@@ -553,6 +553,7 @@ static int pos_layout(asuka_jk_ctx* ctx, const std::vector<int>& col) {
       kpoff[(size_t)k + 1] = kpoff[(size_t)k] + kmax;
     }
     const long long nslot = std::max<long long>(1, kpoff.back() * 64);
+    if (nslot >= (1LL << 31)) return fail(ASUKA_E_INVALID, "ket primitive-pair table exceeds 2^31 slots");
     std::vector<double> k64((size_t)(10 * nslot), 0.0), k32d((size_t)(3 * nslot), 0.0);
     std::vector<float> k32f((size_t)(7 * nslot), 0.0f);
     for (long long k = 0; k < nb; ++k)

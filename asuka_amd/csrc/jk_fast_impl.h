@@ -191,42 +191,44 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 
-// The ket primitive pair in slot t of the transposed tables (DevBasis::kpoff).
+// The ket primitive pair in slot t of the transposed tables (DevBasis::kpoff):
+// the field bases are wave-uniform and the slot a 32-bit per-lane offset, so
+// each field is one load with a scalar base and a vector offset.
 template <typename PPT>
-__device__ __forceinline__ PPT load_ket(const DevBasis& b, long long t);
+__device__ __forceinline__ PPT load_ket(const DevBasis& b, int t);
 template <>
-__device__ __forceinline__ PrimPairF load_ket<PrimPairF>(const DevBasis& b, long long t) {
-  const double* k = b.kp64 + t;
+__device__ __forceinline__ PrimPairF load_ket<PrimPairF>(const DevBasis& b, int t) {
+  const double* k = b.kp64;
   const long long n = b.kslots;
   PrimPairF y;
-  y.eta = k[0];
-  y.inv_eta = k[n];
-  y.ck = k[2 * n];
-  y.Px = k[3 * n];
-  y.Py = k[4 * n];
-  y.Pz = k[5 * n];
-  y.PAx = k[6 * n];
-  y.PAy = k[7 * n];
-  y.PAz = k[8 * n];
-  y.q = k[9 * n];
+  y.eta = k[t];
+  y.inv_eta = (k + n)[t];
+  y.ck = (k + 2 * n)[t];
+  y.Px = (k + 3 * n)[t];
+  y.Py = (k + 4 * n)[t];
+  y.Pz = (k + 5 * n)[t];
+  y.PAx = (k + 6 * n)[t];
+  y.PAy = (k + 7 * n)[t];
+  y.PAz = (k + 8 * n)[t];
+  y.q = (k + 9 * n)[t];
   return y;
 }
 template <>
-__device__ __forceinline__ PrimPairF32 load_ket<PrimPairF32>(const DevBasis& b, long long t) {
-  const double* d = b.kp32d + t;
-  const float* f = b.kp32f + t;
+__device__ __forceinline__ PrimPairF32 load_ket<PrimPairF32>(const DevBasis& b, int t) {
+  const double* d = b.kp32d;
+  const float* f = b.kp32f;
   const long long n = b.kslots;
   PrimPairF32 y;
-  y.Px = d[0];
-  y.Py = d[n];
-  y.Pz = d[2 * n];
-  y.eta = f[0];
-  y.inv_eta = f[n];
-  y.ck = f[2 * n];
-  y.PAx = f[3 * n];
-  y.PAy = f[4 * n];
-  y.PAz = f[5 * n];
-  y.q = f[6 * n];
+  y.Px = d[t];
+  y.Py = (d + n)[t];
+  y.Pz = (d + 2 * n)[t];
+  y.eta = f[t];
+  y.inv_eta = (f + n)[t];
+  y.ck = (f + 2 * n)[t];
+  y.PAx = (f + 3 * n)[t];
+  y.PAy = (f + 4 * n)[t];
+  y.PAz = (f + 5 * n)[t];
+  y.q = (f + 6 * n)[t];
   y.pad = 0.0f;
   return y;
 }
@@ -321,13 +323,13 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   const CT kcut = (nab > 0 && o.eps_prim > 0.0) ? CT(o.eps_prim / (double)ppf[ab0].q) : CT(0);
   // the lane's ket primitive pairs, transposed by position (the lanes' j-th
   // pairs are contiguous): slots ks0 + 64 j
-  const long long ks0 = b.kpoff[uni(pos >> 6)] * 64 + (pos & 63);
+  const int ks0 = (int)b.kpoff[uni(pos >> 6)] * 64 + (pos & 63);
   PPT Yn;
   if (ncd > 0) Yn = load_ket<PPT>(b, ks0);
   for (int j = 0; j < ncd; ++j) {
     const PPT Y = Yn;
     if (Y.q < kcut) break;  // per lane: the rest of its ket pairs are smaller
-    if (j + 1 < ncd) Yn = load_ket<PPT>(b, ks0 + 64LL * (j + 1));  // next ket pair, in flight during this one
+    if (j + 1 < ncd) Yn = load_ket<PPT>(b, ks0 + 64 * (j + 1));  // next ket pair, in flight during this one
     const CT q = CT(Y.eta), iq = CT(Y.inv_eta), ckq = CT(Y.ck);
     const CT QCx = CT(Y.PAx), QCy = CT(Y.PAy), QCz = CT(Y.PAz);
     for (int i = 0; i < nab; ++i) {
