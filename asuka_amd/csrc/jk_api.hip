@@ -647,7 +647,15 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
   if (!(opts->eps_schwarz >= 0.0)) return fail(ASUKA_E_INVALID, "eps_schwarz must be >= 0");
 
   auto* ctx = new asuka_jk_ctx();
-  std::unique_ptr<asuka_jk_ctx> guard(ctx);
+  // on a failed build the context is freed only after the work it queued has
+  // finished (as asuka_jk_ctx_destroy does)
+  struct SyncDelete {
+    void operator()(asuka_jk_ctx* c) const {
+      (void)hipDeviceSynchronize();
+      delete c;
+    }
+  };
+  std::unique_ptr<asuka_jk_ctx, SyncDelete> guard(ctx);
   HIP_CHECK(hipGetDevice(&ctx->device));
   {
     int v = 0;

@@ -22,6 +22,7 @@ solved on the host as the reference does (asuka/hf/df_scf.py:27, :424-446).
 from __future__ import annotations
 
 import os
+import time
 from dataclasses import dataclass
 from typing import Any
 
@@ -242,15 +243,17 @@ def rhf_direct(
         eps, C, D = guess_density(h)
 
     lam = float(damping) if damping else 0.0
+    F_last = None  # the last Fock matrix built before the cycles (sizes the graph's purification)
     if init_fock_cycles > 0 and dm0 is None and mo_coeff0 is None:
         for _ in range(int(init_fock_cycles)):
             Dp = D
-            eps, C, D = guess_density(fock(Dp))
+            F_last = fock(Dp)
+            eps, C, D = guess_density(F_last)
             if lam:
                 D = (1.0 - lam) * D + lam * Dp
 
     if use_graph:
-        return _rhf_graph_iterations(torch, S, h, X, D, jk_ctx, nocc=nocc, occ=occ, enuc=float(enuc),
+        return _rhf_graph_iterations(torch, S, h, X, D, jk_ctx, F_probe=F_last, nocc=nocc, occ=occ, enuc=float(enuc),
                                      max_cycle=int(max_cycle), conv_tol=float(conv_tol),
                                      conv_tol_dm=float(conv_tol_dm), diis=bool(diis), diis_space=int(diis_space),
                                      damping=lam, level_shift=float(level_shift or 0.0),
@@ -353,8 +356,8 @@ def _purify_steps(torch, Fo, nocc: int, max_steps: int = 300) -> int:
     raise RuntimeError("density purification did not converge (no HOMO-LUMO gap?)")
 
 
-def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
-                          diis_space, damping, level_shift, mixed_precision, check_every, profile):
+def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ, enuc, max_cycle, conv_tol,
+                          conv_tol_dm, diis, diis_space, damping, level_shift, mixed_precision, check_every, profile):
     """The SCF cycles of rhf_direct as replays of one captured HIP graph.
 
     One replay is one cycle of asuka/hf/direct_scf.py:337-378 with its state
@@ -380,9 +383,11 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cy
     nslot = max(1, min(int(diis_space), 8)) if diis else 1
     use_fock_out = torch.empty((nao, nao), dtype=f64, device=dev)
 
-    # purification steps: measured on the first Fock matrix, with a margin
-    F0 = direct_fock_rhf(jk_ctx, D0, h, mixed_precision=mixed_precision)
-    npur = int(1.5 * _purify_steps(torch, _sym(X.T @ F0 @ X), nocc)) + 8
+    # purification steps: measured on the last Fock matrix built before the
+    # cycles (the predictor's; else one build of D0), with a margin
+    F0 = F_probe if F_probe is not None else direct_fock_rhf(jk_ctx, D0, h, mixed_precision=mixed_precision)
+    npur = int(1.25 * _purify_steps(torch, _sym(X.T @ F0 @ X), nocc)) + 6
+    nbuild = 1 if F_probe is None else 0  # Fock builds outside the cycles made here
 
     st = {
         "D": D0.clone(), "F": torch.empty((nao, nao), dtype=f64, device=dev),
@@ -400,9 +405,12 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cy
     eye = torch.eye(nao, dtype=f64, device=dev)
     L = _lib.lib()
 
-    def cycle():
+    def cycle(build: bool = True):
         D = st["D"]
-        F = direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed_precision, out=use_fock_out)
+        if build:
+            F = direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed_precision, out=use_fock_out)
+        else:  # warm-up of the other operations: the Fock build needs none
+            F = use_fock_out.copy_(F0)
         if level_shift:
             F = X @ (X.T @ F @ X + level_shift * eye) @ X.T
         if diis:
@@ -444,7 +452,7 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cy
     side = torch.cuda.Stream(device=dev)
     side.wait_stream(torch.cuda.current_stream(dev))
     with torch.cuda.stream(side):
-        cycle()
+        cycle(build=False)
     torch.cuda.current_stream(dev).wait_stream(side)
     torch.cuda.synchronize(dev)
     for k, v in saved.items():
@@ -458,6 +466,8 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cy
     converged = False
     ncyc = 0
     first = None
+    torch.cuda.synchronize(dev)
+    t_loop = time.perf_counter()
     while ncyc < max_cycle and first is None:
         k = min(check_every, max_cycle - ncyc)
         for _ in range(k):
@@ -470,6 +480,8 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cy
         hits = np.nonzero(rec)[0]
         if hits.size:
             first = int(hits[0])
+    t_loop = time.perf_counter() - t_loop  # the replayed cycles (the host's record reads included)
+    nrep = ncyc  # cycles replayed (those after the first converged one are discarded)
     if first is not None:
         converged = True
         ncyc = first + 1
@@ -484,7 +496,9 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, nocc, occ, enuc, max_cy
     eps, C = _gen_eigh(torch, F, X)
     if profile is not None:
         profile.setdefault("scf", {})["iters"] = int(ncyc)
-        profile["scf"]["jk_calls"] = int(ncyc) + 1
+        profile["scf"]["jk_calls"] = int(nrep) + nbuild
+        profile["scf"]["cycles_replayed"] = int(nrep)
+        profile["scf"]["cycle_loop_s"] = t_loop
         profile["scf"]["graph"] = True
         profile["scf"]["purify_steps"] = npur
         profile["scf"]["purify_err_max"] = float(st["recP"][:ncyc].max())

@@ -214,14 +214,22 @@ def run_scf(args, cfg) -> int:
             direct_fock_rhf(ctx, r.dm, hd, mixed_precision=mixed, out=F)
         torch.cuda.synchronize(dev)
         t_fock = time.perf_counter() - t0
+        # the replayed cycles alone: their time against that of as many Fock builds
+        nrep = int(prof["scf"].get("cycles_replayed", 0))
+        t_loop = float(prof["scf"].get("cycle_loop_s", 0.0))
+        t_fock_loop = t_fock * nrep / max(1, nfock)
         out[mode] = {"s": t_scf, "e_tot": r.e_tot, "iters": r.niter, "fock_builds": nfock, "fock_s": t_fock,
                      "non_fock_fraction": (t_scf - t_fock) / t_scf, "graph": bool(prof["scf"].get("graph")),
+                     "cycles_replayed": nrep, "cycle_loop_s": t_loop,
+                     "non_fock_fraction_cycles": (t_loop - t_fock_loop) / t_loop if t_loop > 0 else None,
                      "purify_steps": prof["scf"].get("purify_steps"),
                      "purify_err_max": prof["scf"].get("purify_err_max")}
         if mode == "fp64":
             D15 = r.dm
         log(f"{mode}: {niter} SCF iterations in {t_scf:.2f} s ({nfock} Fock builds alone {t_fock:.2f} s, "
-            f"outside the builds {100 * (t_scf - t_fock) / t_scf:.1f} %), E = {r.e_tot:.10f}")
+            f"outside the builds {100 * (t_scf - t_fock) / t_scf:.1f} %; the {nrep} graph-replayed cycles "
+            f"{t_loop:.2f} s, outside their builds {100 * (t_loop - t_fock_loop) / max(t_loop, 1e-9):.1f} %), "
+            f"E = {r.e_tot:.10f}")
     dE = out["mixed"]["e_tot"] - out["fp64"]["e_tot"]
     # precision of one build at the FP64 run's iteration-15 density:
     # E = 1/2 sum D (h + F), so dE = 1/2 sum D (F_mixed - F_fp64)
