@@ -184,11 +184,7 @@ __device__ __forceinline__ bool wave_all(bool v) { return __builtin_amdgcn_ballo
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Sum over the 64 lanes of the wavefront (all lanes must be converged).
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_total(v); }
 
 
 // The ket primitive pair in slot t of the transposed tables (DevBasis::kpoff):

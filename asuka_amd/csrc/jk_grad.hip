@@ -61,11 +61,7 @@ __device__ inline void cart_of(int l, int i, int& x, int& y) {
   y = l - xx - rem;
 }
 
-__device__ inline double wsum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
+__device__ inline double wsum(double v) { return wave_total(v); }
 
 struct GradArgs {
   const double* D;

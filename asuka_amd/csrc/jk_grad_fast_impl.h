@@ -55,11 +55,7 @@ constexpr double binom_g(int n, int k) {
   return r;
 }
 
-__device__ __forceinline__ double wave_sum_g(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
+__device__ __forceinline__ double wave_sum_g(double v) { return wave_total(v); }
 __device__ __forceinline__ int uni_g(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <int LA, int LB, int LC, int LD>

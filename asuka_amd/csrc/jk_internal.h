@@ -241,6 +241,40 @@ __device__ inline T* shifted(T* p, long long sh) {
 
 __host__ __device__ constexpr inline int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
 
+// v of the lane that DPP control CTRL selects (both 32-bit halves moved);
+// lanes of rows outside ROWS receive 0.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+  int lo, hi;
+  if constexpr (ROWS == 0xF) {  // every row written: no old value to keep
+    lo = __builtin_amdgcn_mov_dpp((int)(unsigned)x, CTRL, ROWS, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(x >> 32), CTRL, ROWS, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, ROWS, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, ROWS, 0xF, false);
+  }
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum of v over the 64 lanes of the wavefront, the same on every lane; all
+// lanes must be enabled. The lanes are summed in registers with DPP moves, not
+// through the LDS crossbar (ds_bpermute): within each 16-lane row by xor-1 and
+// xor-2 quad permutes and the half-row and row mirrors, then across rows by
+// the row 15 and row 31 broadcasts, which leave the total on lane 63.
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror: lane i <- 7 - i within 8
+  v += dpp_f64<0x140>(v);  // row_mirror: lane i <- 15 - i within 16
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Decode work index k of a class pair into (bra pair, ket pair); false for
 // segment padding.
 __device__ inline bool decode_work(const ClassWork& w, long long k, int& P, int& Qp) {
