@@ -244,9 +244,12 @@ def rhf_direct(
 
     lam = float(damping) if damping else 0.0
     F_last = None  # the last Fock matrix built before the cycles (sizes the graph's purification)
+    keep = profile is not None and bool(profile.get("keep_densities"))
     if init_fock_cycles > 0 and dm0 is None and mo_coeff0 is None:
         for _ in range(int(init_fock_cycles)):
             Dp = D
+            if keep:
+                profile.setdefault("scf", {}).setdefault("predictor_densities", []).append(Dp.clone())
             F_last = fock(Dp)
             eps, C, D = guess_density(F_last)
             if lam:
@@ -466,6 +469,11 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
     converged = False
     ncyc = 0
     first = None
+    # profile["keep_densities"]: keep each cycle's input density (a device copy
+    # per cycle, 2 nao^2 doubles of traffic) so that the Fock builds of the
+    # cycles can be timed alone on the same densities
+    keep = profile is not None and bool(profile.get("keep_densities"))
+    dens = [st["D"].clone()] if keep else None
     torch.cuda.synchronize(dev)
     t_loop = time.perf_counter()
     while ncyc < max_cycle and first is None:
@@ -475,6 +483,8 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
                 graph.replay()
             else:
                 cycle()
+            if keep:
+                dens.append(st["D"].clone())
         ncyc += k
         rec = st["recC"][:ncyc].cpu().numpy()
         hits = np.nonzero(rec)[0]
@@ -498,6 +508,8 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
         profile.setdefault("scf", {})["iters"] = int(ncyc)
         profile["scf"]["jk_calls"] = int(nrep) + nbuild
         profile["scf"]["cycles_replayed"] = int(nrep)
+        if keep:
+            profile["scf"]["cycle_densities"] = dens[:nrep]
         profile["scf"]["cycle_loop_s"] = t_loop
         profile["scf"]["graph"] = True
         profile["scf"]["purify_steps"] = npur

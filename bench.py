@@ -171,7 +171,11 @@ def run_scf(args, cfg) -> int:
     captured HIP graph (Fock build, device DIIS, purified density, energy and
     convergence test); the energy after 15 iterations of each is compared. The
     time outside the Fock builds is the SCF time minus that of the same number
-    of Fock builds alone."""
+    of Fock builds alone on the same densities: the SCF keeps the input density
+    of every Fock build it makes (the predictor's and each cycle's), and each is
+    built again alone, timed with a device synchronisation on both sides (the
+    work of a build depends on its density through the density screening and
+    the FP32 routing, so builds at another density would misstate it)."""
     import torch
 
     from asuka_amd.frontend import build_int1e
@@ -201,23 +205,32 @@ def run_scf(args, cfg) -> int:
         # warm-up: one short SCF (graph capture, allocator pools)
         rhf_direct(ints.S, ints.hcore, ctx, max_cycle=2, conv_tol=0.0, conv_tol_dm=0.0, mixed_precision=mixed, **kw)
         torch.cuda.synchronize(dev)
-        prof = {}
+        prof = {"keep_densities": True}
         t0 = time.perf_counter()
         r = rhf_direct(ints.S, ints.hcore, ctx, max_cycle=niter, conv_tol=0.0, conv_tol_dm=0.0,
                        mixed_precision=mixed, profile=prof, **kw)
         torch.cuda.synchronize(dev)
         t_scf = time.perf_counter() - t0
-        nfock = int(prof["scf"]["jk_calls"]) + 1  # + the predictor cycle
+        d_cyc = prof["scf"].pop("cycle_densities", [])
+        d_pre = prof["scf"].pop("predictor_densities", [])
         F = torch.empty_like(hd)
-        t0 = time.perf_counter()
-        for _ in range(nfock):
-            direct_fock_rhf(ctx, r.dm, hd, mixed_precision=mixed, out=F)
-        torch.cuda.synchronize(dev)
-        t_fock = time.perf_counter() - t0
-        # the replayed cycles alone: their time against that of as many Fock builds
+
+        def fock_alone(dens):
+            t = 0.0
+            for Dk in dens:
+                torch.cuda.synchronize(dev)
+                t1 = time.perf_counter()
+                direct_fock_rhf(ctx, Dk, hd, mixed_precision=mixed, out=F)
+                torch.cuda.synchronize(dev)
+                t += time.perf_counter() - t1
+            return t
+
+        t_fock_loop = fock_alone(d_cyc)  # the replayed cycles' builds alone
+        t_fock = t_fock_loop + fock_alone(d_pre)  # every build of the SCF alone
+        nfock = len(d_cyc) + len(d_pre)
+        del d_cyc, d_pre
         nrep = int(prof["scf"].get("cycles_replayed", 0))
         t_loop = float(prof["scf"].get("cycle_loop_s", 0.0))
-        t_fock_loop = t_fock * nrep / max(1, nfock)
         out[mode] = {"s": t_scf, "e_tot": r.e_tot, "iters": r.niter, "fock_builds": nfock, "fock_s": t_fock,
                      "non_fock_fraction": (t_scf - t_fock) / t_scf, "graph": bool(prof["scf"].get("graph")),
                      "cycles_replayed": nrep, "cycle_loop_s": t_loop,
@@ -227,7 +240,7 @@ def run_scf(args, cfg) -> int:
         if mode == "fp64":
             D15 = r.dm
         log(f"{mode}: {niter} SCF iterations in {t_scf:.2f} s ({nfock} Fock builds alone {t_fock:.2f} s, "
-            f"outside the builds {100 * (t_scf - t_fock) / t_scf:.1f} %; the {nrep} graph-replayed cycles "
+            f"same densities, outside the builds {100 * (t_scf - t_fock) / t_scf:.1f} %; the {nrep} graph-replayed cycles "
             f"{t_loop:.2f} s, outside their builds {100 * (t_loop - t_fock_loop) / max(t_loop, 1e-9):.1f} %), "
             f"E = {r.e_tot:.10f}")
     dE = out["mixed"]["e_tot"] - out["fp64"]["e_tot"]
