@@ -34,7 +34,7 @@ int eri_fast_nch(int la, int lb, int lc, int ld);
 static thread_local std::string g_err;
 
 // default FP32 threshold on Q_ab Q_cd max|D| for builds with ASUKA_JK_MIXED
-static constexpr double kDefaultTauMixed = 1e-7;
+static constexpr double kDefaultTauMixed = 1e-5;
 // Row tasks: at most this many wave units of one bra row per workgroup.
 static constexpr int kTaskUnits = 32;
 static constexpr int kTaskItems = 256;  // wave items (unit x chunk) per row task, for split plans

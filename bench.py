@@ -505,7 +505,7 @@ def main() -> int:
     ap.add_argument("--fd", action="store_true", help="gradient workload: central-difference check on 6 coordinates")
     ap.add_argument("--autotune", action="store_true", help="J/K workloads: autotune launch shapes before timing")
     ap.add_argument("--scf", action="store_true", help="config 5: time 15 SCF iterations, FP64 and mixed")
-    ap.add_argument("--tau", type=float, default=1e-7, help="FP32 cut on Q_ab Q_cd max|D| of the mixed build")
+    ap.add_argument("--tau", type=float, default=1e-5, help="FP32 cut on Q_ab Q_cd max|D| of the mixed build")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (the calibration kernel of the HBM counters)")
     args = ap.parse_args()

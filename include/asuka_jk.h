@@ -78,7 +78,7 @@ typedef struct {
   double tau_mixed;   /* with ASUKA_JK_MIXED: a wavefront of quartets whose Q_ab*Q_cd*max|D|
                          bounds are all below tau_mixed runs the primitive setup, Rys roots,
                          recursion and transfer in FP32 (J/K accumulation stays FP64);
-                         <= 0 => 1e-7 */
+                         <= 0 => 1e-5 */
   int32_t streams;    /* a build's class-pair launches are spread over this many streams: the
                          caller's and streams - 1 of the context's own, forked from and joined
                          back into the caller's stream by events (stream order and graph capture

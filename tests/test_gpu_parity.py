@@ -232,7 +232,7 @@ def test_config2_size_properties(gpu):
 
 @pytest.mark.parametrize("case", ["h2o4_dz", "h2o2_tz"])
 def test_mixed_precision_jk(gpu, jk_golden, case):
-    """Mixed FP32/FP64 mode. With the default bound cut (1e-7) J and K stay
+    """Mixed FP32/FP64 mode. With the default bound cut (1e-5) J and K stay
     within 1e-8 of the FP64 oracle; with every quartet forced onto the FP32
     kernels (cut 1e3) they stay within 1e-5, which no index or layout mistake
     in the FP32 instantiations would meet, and differ from FP64 (so the FP32
