@@ -28,12 +28,37 @@ from typing import Any
 
 import numpy as np
 
-from .direct_jk import DirectJKContext, direct_fock_rhf, direct_JK, direct_JK_multi
+from .direct_jk import DirectJKContext, _flags, direct_fock_rhf, direct_JK, direct_JK_multi
 
 _CPU_EIGH_MAX_NAO = int(os.environ.get("ASUKA_HF_CPU_EIGH_MAX_NAO", "96") or 96)
 # False runs the captured SCF cycle eagerly instead of replaying its graph
 # (the same operations; tests compare the two)
 _GRAPH_REPLAY = True
+
+# Mixed-precision SCF: a mixed build routes each wave unit of quartets by its
+# bound Q_ab Q_cd max|D| against tau, so from one cycle to the next units change
+# precision with the density and the Fock matrix moves by up to the FP32 error
+# (about 1e-9 at tau = 1e-5 for (H2O)64 cc-pVDZ). That stalls a tight
+# convergence test, so once a cycle's ||D_new - D||_F falls below this the
+# cycles build in FP64. DIIS keeps its history (restarting it there cost
+# tens of cycles at (H2O)64 cc-pVDZ); the switch comes early enough that
+# FP64 Fock matrices have replaced the mixed ones in its ring of 8 by
+# convergence.
+_MIXED_UNTIL_DM = 1e-3
+
+
+def _mixed_schedule(mixed_precision, mixed_until_dm) -> tuple[bool, float]:
+    """(mixed builds at the start, ||D_new - D||_F below which the builds turn
+    FP64; 0 keeps them mixed to the end). mixed_until_dm None: the
+    environment's ASUKA_SCF_MIXED_UNTIL_DM, else _MIXED_UNTIL_DM."""
+    from .. import _lib
+
+    if not (_flags(mixed_precision) & _lib.MIXED):
+        return False, 0.0
+    if mixed_until_dm is None:
+        env = os.environ.get("ASUKA_SCF_MIXED_UNTIL_DM", "").strip()
+        mixed_until_dm = float(env) if env else _MIXED_UNTIL_DM
+    return True, max(0.0, float(mixed_until_dm))
 
 
 @dataclass(frozen=True)
@@ -167,6 +192,7 @@ def rhf_direct(
     use_graph: bool | None = None,
     check_every: int = 4,
     fock_fn=None,
+    mixed_until_dm: float | None = None,
 ) -> SCFResult:
     """RHF with integral-direct J/K (asuka/hf/direct_scf.py:230-402).
 
@@ -175,7 +201,14 @@ def rhf_direct(
     cycles is one replay of a captured HIP graph (Fock build, DIIS with the
     device solve, density by purification, energy and convergence test), and
     the host reads the convergence record every ``check_every`` iterations
-    (:func:`_rhf_graph_iterations`)."""
+    (:func:`_rhf_graph_iterations`).
+
+    With mixed-precision builds (``mixed_precision``, default
+    ``$ASUKA_ERI_MIXED_PRECISION`` or on) the cycles switch to FP64 builds
+    once ||D_new - D||_F falls below ``mixed_until_dm`` (default
+    ``$ASUKA_SCF_MIXED_UNTIL_DM`` or 1e-3; 0 keeps them mixed): near
+    convergence the density-dependent FP32 routing would move the Fock matrix
+    by its FP32 error from cycle to cycle."""
     import torch
 
     dev = jk_ctx.device if jk_ctx is not None else torch.device("cuda", torch.cuda.current_device())
@@ -197,13 +230,15 @@ def rhf_direct(
     X = _orthogonalizer(torch, S)
 
     use_fock = fock_fn is None and int(jk_ctx.max_l) <= 2
+    mixed0, mixed_until = _mixed_schedule(mixed_precision, mixed_until_dm)
+    prec = {"mixed": mixed0}  # the builds' precision now (the eager cycles switch it)
 
     def fock(D):
         if fock_fn is not None:  # another two-electron backend (density fitting)
             return _sym(fock_fn(D))
         if use_fock:
-            return direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed_precision)
-        J, K = direct_JK(jk_ctx, D, mixed_precision=mixed_precision)
+            return direct_fock_rhf(jk_ctx, D, h, mixed_precision=prec["mixed"])
+        J, K = direct_JK(jk_ctx, D, mixed_precision=prec["mixed"])
         return _sym(h + J - 0.5 * K)
 
     def shifted(F):
@@ -260,8 +295,8 @@ def rhf_direct(
                                      max_cycle=int(max_cycle), conv_tol=float(conv_tol),
                                      conv_tol_dm=float(conv_tol_dm), diis=bool(diis), diis_space=int(diis_space),
                                      damping=lam, level_shift=float(level_shift or 0.0),
-                                     mixed_precision=mixed_precision, check_every=max(1, int(check_every)),
-                                     profile=profile)
+                                     mixed_precision=mixed0, mixed_until=mixed_until,
+                                     check_every=max(1, int(check_every)), profile=profile)
 
     diis_obj = _DIIS(torch, diis_space) if diis else None
     e_last = None
@@ -269,6 +304,7 @@ def rhf_direct(
     cycle = 0
     njk = 0
     energies, dm_errors = [], []
+    fp64_from = None  # first cycle with an FP64 build after the switch
     for cycle in range(1, int(max_cycle) + 1):
         F = shifted(fock(D))
         njk += 1
@@ -296,9 +332,13 @@ def rhf_direct(
         if de < conv_tol and dm_err < conv_tol_dm:
             converged = True
             break
+        if prec["mixed"] and dm_err < mixed_until:
+            prec["mixed"] = False
+            fp64_from = cycle + 1
     if profile is not None:
         profile.setdefault("scf", {})["iters"] = int(cycle)
         profile["scf"]["jk_calls"] = njk
+        profile["scf"]["fp64_from_cycle"] = fp64_from
         profile["scf"]["graph"] = False
         profile["scf"]["energies"] = energies
         profile["scf"]["dm_errors"] = dm_errors
@@ -360,7 +400,8 @@ def _purify_steps(torch, Fo, nocc: int, max_steps: int = 300) -> int:
 
 
 def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ, enuc, max_cycle, conv_tol,
-                          conv_tol_dm, diis, diis_space, damping, level_shift, mixed_precision, check_every, profile):
+                          conv_tol_dm, diis, diis_space, damping, level_shift, mixed_precision, check_every, profile,
+                          mixed_until=0.0):
     """The SCF cycles of rhf_direct as replays of one captured HIP graph.
 
     One replay is one cycle of asuka/hf/direct_scf.py:337-378 with its state
@@ -408,10 +449,10 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
     eye = torch.eye(nao, dtype=f64, device=dev)
     L = _lib.lib()
 
-    def cycle(build: bool = True):
+    def cycle(build: bool = True, mixed: bool = bool(mixed_precision)):
         D = st["D"]
         if build:
-            F = direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed_precision, out=use_fock_out)
+            F = direct_fock_rhf(jk_ctx, D, h, mixed_precision=mixed, out=use_fock_out)
         else:  # warm-up of the other operations: the Fock build needs none
             F = use_fock_out.copy_(F0)
         if level_shift:
@@ -461,11 +502,19 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
     for k, v in saved.items():
         st[k].copy_(v)
     del saved
-    graph = None
+    # mixed builds switching to FP64 (mixed_until > 0): one graph per precision,
+    # each with its own memory pool (a shared pool is only safe for graphs
+    # replayed in the order they were captured; the FP64 graph replays alone)
+    switch = bool(mixed_precision) and mixed_until > 0.0
+    graphs = {}
     if _GRAPH_REPLAY:
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
-            cycle()
+        for mp in ((True, False) if switch else (bool(mixed_precision),)):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                cycle(mixed=mp)
+            graphs[mp] = g
+    mixed_now = bool(mixed_precision)
+    fp64_from = None  # first cycle (1-based) with an FP64 build after the switch
     converged = False
     ncyc = 0
     first = None
@@ -477,12 +526,15 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
     torch.cuda.synchronize(dev)
     t_loop = time.perf_counter()
     while ncyc < max_cycle and first is None:
-        k = min(check_every, max_cycle - ncyc)
+        # while the builds are mixed and may switch, the record is read after
+        # every cycle, so the switch comes at the cycle the eager driver would
+        # take (one scalar read per cycle against a cycle of milliseconds)
+        k = 1 if (mixed_now and switch) else min(check_every, max_cycle - ncyc)
         for _ in range(k):
-            if graph is not None:
-                graph.replay()
+            if graphs:
+                graphs[mixed_now].replay()
             else:
-                cycle()
+                cycle(mixed=mixed_now)
             if keep:
                 dens.append(st["D"].clone())
         ncyc += k
@@ -490,6 +542,9 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
         hits = np.nonzero(rec)[0]
         if hits.size:
             first = int(hits[0])
+        elif mixed_now and switch and float(st["recD"][ncyc - 1]) < mixed_until:
+            mixed_now = False
+            fp64_from = ncyc + 1  # 1-based, as in the eager driver
     t_loop = time.perf_counter() - t_loop  # the replayed cycles (the host's record reads included)
     nrep = ncyc  # cycles replayed (those after the first converged one are discarded)
     if first is not None:
@@ -508,6 +563,7 @@ def _rhf_graph_iterations(torch, S, h, X, D0, jk_ctx, *, F_probe=None, nocc, occ
         profile.setdefault("scf", {})["iters"] = int(ncyc)
         profile["scf"]["jk_calls"] = int(nrep) + nbuild
         profile["scf"]["cycles_replayed"] = int(nrep)
+        profile["scf"]["fp64_from_cycle"] = fp64_from
         if keep:
             profile["scf"]["cycle_densities"] = dens[:nrep]
         profile["scf"]["cycle_loop_s"] = t_loop
@@ -552,7 +608,8 @@ def _roothaan_fock(torch, Fa, Fb, Da, Db, S):
 
 
 def _open_shell_scf(kind: str, S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
-                    diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile):
+                    diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile,
+                    mixed_until_dm=None):
     import torch
 
     dev = jk_ctx.device
@@ -600,8 +657,9 @@ def _open_shell_scf(kind: str, S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle,
     converged = False
     cycle = 0
     njk = 0
+    mixed_now, mixed_until = _mixed_schedule(mixed_precision, mixed_until_dm)
     for cycle in range(1, int(max_cycle) + 1):
-        Ja, Ka, Jb, Kb = direct_JK_multi(jk_ctx, Da, Db, mixed_precision=mixed_precision)
+        Ja, Ka, Jb, Kb = direct_JK_multi(jk_ctx, Da, Db, mixed_precision=mixed_now)
         njk += 1
         J = Ja + Jb
         Fa = _sym(h + J - Ka)
@@ -645,6 +703,8 @@ def _open_shell_scf(kind: str, S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle,
         if de < conv_tol and dm_err < conv_tol_dm:
             converged = True
             break
+        if mixed_now and dm_err < mixed_until:  # FP64 builds from here (see _MIXED_UNTIL_DM)
+            mixed_now = False
     if profile is not None:
         profile.setdefault("scf", {})["iters"] = int(cycle)
         profile["scf"]["jk_calls"] = njk
@@ -659,23 +719,27 @@ def _open_shell_scf(kind: str, S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle,
 def uhf_direct(S, hcore, jk_ctx: DirectJKContext, *, nalpha: int, nbeta: int, enuc: float = 0.0,
                max_cycle: int = 50, conv_tol: float = 1e-10, conv_tol_dm: float = 1e-8, diis: bool = True,
                diis_start_cycle: int = 2, diis_space: int = 8, damping: float = 0.0, dm0=None, mo_coeff0=None,
-               mixed_precision: bool | None = None, profile: dict | None = None) -> SCFResult:
+               mixed_precision: bool | None = None, profile: dict | None = None,
+               mixed_until_dm: float | None = None) -> SCFResult:
     """UHF with the two-density direct J/K build (asuka/hf/direct_scf.py:405-573):
     core guess, per-spin Fock h + J_a + J_b - K_s, one DIIS per spin on
     F D S - (F D S)^T, per-spin eigh, D_s = C_s,occ C_s,occ^T."""
     return _open_shell_scf("UHF", S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
-                           diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile)
+                           diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile,
+                           mixed_until_dm)
 
 
 def rohf_direct(S, hcore, jk_ctx: DirectJKContext, *, nalpha: int, nbeta: int, enuc: float = 0.0,
                 max_cycle: int = 50, conv_tol: float = 1e-10, conv_tol_dm: float = 1e-8, diis: bool = True,
                 diis_start_cycle: int = 2, diis_space: int = 8, damping: float = 0.0, dm0=None, mo_coeff0=None,
-                mixed_precision: bool | None = None, profile: dict | None = None) -> SCFResult:
+                mixed_precision: bool | None = None, profile: dict | None = None,
+                mixed_until_dm: float | None = None) -> SCFResult:
     """ROHF with the two-density direct J/K build (asuka/hf/direct_scf.py:576-):
     Roothaan's effective Fock matrix from the spin Fock matrices, one DIIS on
     F D_tot S - (F D_tot S)^T, one eigh, shared orbitals for both spins."""
     return _open_shell_scf("ROHF", S, hcore, jk_ctx, nalpha, nbeta, enuc, max_cycle, conv_tol, conv_tol_dm, diis,
-                           diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile)
+                           diis_start_cycle, diis_space, damping, dm0, mo_coeff0, mixed_precision, profile,
+                           mixed_until_dm)
 
 
 __all__ = ["SCFResult", "rhf_direct", "rohf_direct", "uhf_direct"]

@@ -40,7 +40,7 @@ METRIC = "Fock-build s & quartets/s: mixed vs FP64 GPU at ≤1e-6 Eh; gradient s
 # by what the counter reads for a 1 GiB device copy), measured with rocprofv3
 # --pmc per workload and mode (profiles/*_hbm_traffic.md); None where not
 # measured.
-HBM_TRAFFIC = {(2, "fp64"): 1.295e10, (2, "mixed"): 1.711e10, (5, "fp64"): 4.550e11, (5, "mixed"): 4.488e11}
+HBM_TRAFFIC = {(2, "fp64"): 1.295e10, (2, "mixed"): 1.711e10, (5, "fp64"): 4.385e11, (5, "mixed"): 4.325e11}
 
 CONFIGS = {
     2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12, kind="jk",
@@ -189,7 +189,7 @@ def run_scf(args, cfg) -> int:
     mol = water_cluster(cfg["n"], cfg["seed"], cfg["basis"])
     ao, _ = build_ao_basis_cart(mol)
     t0 = time.perf_counter()
-    ctx = make_direct_jk_context(ao, eps_schwarz=cfg["eps"], atom_coords_bohr=mol.coords_bohr,
+    ctx = make_direct_jk_context(ao, eps_schwarz=cfg["eps"], tau_mixed=args.tau, atom_coords_bohr=mol.coords_bohr,
                                  atom_charges=mol.charges)
     t_ctx = time.perf_counter() - t0
     ints = build_int1e(ctx)
@@ -249,13 +249,17 @@ def run_scf(args, cfg) -> int:
     F64 = direct_fock_rhf(ctx, D15, hd, mixed_precision=False)
     F32 = direct_fock_rhf(ctx, D15, hd, mixed_precision=True)
     dE_build = float(0.5 * (D15 * (F32 - F64)).sum())
-    # converged energies, both modes continued from that density
+    # converged energies, both modes continued from that density (the mixed
+    # SCF builds in FP64 once ||D_new - D||_F < 1e-3, the driver's default)
     conv = {}
     for mode in ("fp64", "mixed"):
+        pc = {}
         r = rhf_direct(ints.S, ints.hcore, ctx, max_cycle=60, conv_tol=1e-9, conv_tol_dm=1e-6, dm0=D15,
-                       mixed_precision=(mode == "mixed"), **kw)
-        conv[mode] = {"e_tot": r.e_tot, "iters": r.niter, "converged": bool(r.converged)}
-        log(f"{mode}: converged={r.converged} after {r.niter} more iterations, E = {r.e_tot:.10f}")
+                       mixed_precision=(mode == "mixed"), profile=pc, **kw)
+        conv[mode] = {"e_tot": r.e_tot, "iters": r.niter, "converged": bool(r.converged),
+                      "fp64_builds_from_cycle": pc["scf"].get("fp64_from_cycle")}
+        log(f"{mode}: converged={r.converged} after {r.niter} more iterations, E = {r.e_tot:.10f}"
+            + (f" (FP64 builds from cycle {conv[mode]['fp64_builds_from_cycle']})" if mode == "mixed" else ""))
     dE_conv = conv["mixed"]["e_tot"] - conv["fp64"]["e_tot"]
     res = {
         "metric": "direct RHF: seconds for 15 SCF iterations, mixed vs FP64 GPU build",
@@ -271,7 +275,8 @@ def run_scf(args, cfg) -> int:
         "dtype": "f64+f32",
         "data": "synthetic (seeded water cluster geometry)",
         "config": {"workload": cfg["workload_scf"], "nao": ao.nao, "basis": cfg["basis"], "n_water": cfg["n"],
-                   "geometry_seed": cfg["seed"], "eps_schwarz": cfg["eps"], "iterations": niter},
+                   "geometry_seed": cfg["seed"], "eps_schwarz": cfg["eps"], "iterations": niter,
+                   "tau_mixed": args.tau},
         "fp64_s": out["fp64"]["s"],
         "mixed_s": out["mixed"]["s"],
         "speedup_mixed_vs_fp64": out["fp64"]["s"] / out["mixed"]["s"],

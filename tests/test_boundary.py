@@ -76,3 +76,24 @@ def test_host_table_moments_dense_scan(golden):
             F = mmd_ref.boys_array(2 * n - 1, float(T))
             for k in range(2 * n):
                 assert abs(np.sum(w * x ** k) - F[k]) <= 4e-15 * F[0], (n, T, k)
+
+
+def test_scf_mixed_precision_schedule(monkeypatch):
+    """The SCF drivers' precision schedule: FP64 when the build is not mixed;
+    mixed builds switch to FP64 below ||dD||_F = 1e-3 by default, overridden
+    by ASUKA_SCF_MIXED_UNTIL_DM or the argument; 0 keeps them mixed."""
+    from asuka_amd.hf.direct_scf import _mixed_schedule
+
+    monkeypatch.delenv("ASUKA_SCF_MIXED_UNTIL_DM", raising=False)
+    assert _mixed_schedule(False, None) == (False, 0.0)
+    assert _mixed_schedule(False, 1e-2) == (False, 0.0)
+    assert _mixed_schedule(True, None) == (True, 1e-3)
+    assert _mixed_schedule(True, 5e-4) == (True, 5e-4)
+    assert _mixed_schedule(True, 0.0) == (True, 0.0)
+    monkeypatch.setenv("ASUKA_SCF_MIXED_UNTIL_DM", "2e-3")
+    assert _mixed_schedule(True, None) == (True, 2e-3)
+    assert _mixed_schedule(True, 1e-4) == (True, 1e-4)
+    monkeypatch.setenv("ASUKA_ERI_MIXED_PRECISION", "0")
+    assert _mixed_schedule(None, None) == (False, 0.0)
+    monkeypatch.setenv("ASUKA_ERI_MIXED_PRECISION", "1")
+    assert _mixed_schedule(None, None) == (True, 2e-3)

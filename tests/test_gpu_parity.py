@@ -259,14 +259,31 @@ def test_mixed_precision_jk(gpu, jk_golden, case):
 
 
 def test_mixed_precision_scf_energy(gpu):
-    """RHF energy of (H2O)2 cc-pVDZ with the mixed-precision build stays within
-    1e-6 Eh of the FP64 build."""
+    """RHF energy of (H2O)2 cc-pVDZ with the mixed-precision build: with the
+    driver's switch to FP64 builds once ||D_new - D||_F < 1e-3 (graph and
+    eager drivers) the converged energy is the FP64 one to 1e-9 Eh; with the
+    builds mixed to the end it stays within 1e-6 Eh."""
     from asuka_amd.frontend import run_hf_df
 
     mol = water_cluster(2, 3, "cc-pVDZ")
-    e64 = run_hf_df(mol, two_e_backend="direct", eps_schwarz=1e-12)
-    emx = run_hf_df(mol, two_e_backend="direct", eps_schwarz=1e-12, mixed_precision=True)
-    assert e64.scf.converged and emx.scf.converged
+    e64 = run_hf_df(mol, two_e_backend="direct", eps_schwarz=1e-12, mixed_precision=False)
+    assert e64.scf.converged
+    for use_graph in (True, False):
+        prof = {}
+        emx = run_hf_df(mol, two_e_backend="direct", eps_schwarz=1e-12, mixed_precision=True, use_graph=use_graph,
+                        profile=prof)
+        assert emx.scf.converged and prof["scf"]["graph"] is use_graph
+        assert prof["scf"]["fp64_from_cycle"] is not None and prof["scf"]["fp64_from_cycle"] > 1
+        assert abs(emx.scf.e_tot - e64.scf.e_tot) < 1e-9, (use_graph, emx.scf.e_tot - e64.scf.e_tot)
+    import os
+
+    os.environ["ASUKA_SCF_MIXED_UNTIL_DM"] = "0"
+    try:
+        prof = {}
+        emx = run_hf_df(mol, two_e_backend="direct", eps_schwarz=1e-12, mixed_precision=True, profile=prof)
+    finally:
+        del os.environ["ASUKA_SCF_MIXED_UNTIL_DM"]
+    assert emx.scf.converged and prof["scf"]["fp64_from_cycle"] is None
     assert abs(emx.scf.e_tot - e64.scf.e_tot) < 1e-6
 
 
