@@ -554,7 +554,7 @@ static int pos_layout(asuka_jk_ctx* ctx, const std::vector<int>& col) {
     }
     const long long nslot = std::max<long long>(1, kpoff.back() * 64);
     if (nslot >= (1LL << 31)) return fail(ASUKA_E_INVALID, "ket primitive-pair table exceeds 2^31 slots");
-    std::vector<double> k64((size_t)(10 * nslot), 0.0), k32d((size_t)(3 * nslot), 0.0);
+    std::vector<double> k64((size_t)(11 * nslot), 0.0), k32d((size_t)(3 * nslot), 0.0);
     std::vector<float> k32f((size_t)(7 * nslot), 0.0f);
     for (long long k = 0; k < nb; ++k)
       for (int t = 0; t < 64; ++t) {
@@ -562,9 +562,10 @@ static int pos_layout(asuka_jk_ctx* ctx, const std::vector<int>& col) {
         if (sp < 0) continue;
         for (int j = 0; j < ctx->pp_count[sp]; ++j) {
           const PrimPairF& f = ctx->ppf[(size_t)ctx->pp_start[sp] + j];
+          const double a_exp = ctx->pp[(size_t)ctx->pp_start[sp] + j].a_exp;
           const size_t slot = (size_t)((kpoff[(size_t)k] + j) * 64 + t), n = (size_t)nslot;
-          const double v64[10] = {f.eta, f.inv_eta, f.ck, f.Px, f.Py, f.Pz, f.PAx, f.PAy, f.PAz, f.q};
-          for (int x = 0; x < 10; ++x) k64[x * n + slot] = v64[x];
+          const double v64[11] = {f.eta, f.inv_eta, f.ck, f.Px, f.Py, f.Pz, f.PAx, f.PAy, f.PAz, f.q, a_exp};
+          for (int x = 0; x < 11; ++x) k64[x * n + slot] = v64[x];
           k32d[0 * n + slot] = f.Px;
           k32d[1 * n + slot] = f.Py;
           k32d[2 * n + slot] = f.Pz;

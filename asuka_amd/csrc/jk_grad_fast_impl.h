@@ -56,6 +56,7 @@ constexpr double binom_g(int n, int k) {
 }
 
 __device__ __forceinline__ double wave_sum_g(double v) { return wave_total(v); }
+constexpr double kSqrt2Pi52g = 5.91496717279561287782;  // sqrt(2 pi^(5/2))
 __device__ __forceinline__ int uni_g(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <int LA, int LB, int LC, int LD>
@@ -140,10 +141,11 @@ struct GradFastArgs {
   int natm;
 };
 
-// One wave unit, components [C0, C1) of the class (one chunk).
+// One wave unit, components [C0, C1) of the class (one chunk). pos: the
+// lane's ket position (contiguous across the lanes; DevBasis::kpoff).
 template <int LA, int LB, int LC, int LD, int C0, int C1>
 __device__ __forceinline__ void deriv_unit(const DevBasis& b, const GradFastArgs& g, const double* tab, int P, int Qp,
-                                           bool active, double* sgrad, bool lds_grad) {
+                                           int pos, bool active, double* sgrad, bool lds_grad) {
   using DP = DPlan<LA, LB, LC, LD>;
   constexpr int NB = DP::NB, NC = DP::NC, ND = DP::ND;
   constexpr int NR = DP::NR, NM = DP::NMAX, MM = DP::MMAX;
@@ -190,18 +192,41 @@ __device__ __forceinline__ void deriv_unit(const DevBasis& b, const GradFastArgs
   }
   double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
   const int ab0 = uni_g(b.pp_start[P]), nab = uni_g(b.pp_count[P]);
-  const int cd0 = b.pp_start[Qp], ncd = active ? b.pp_count[Qp] : 0;
+  const int ncd = active ? b.kpp[pos].y : 0;
+  // the lane's ket primitive pairs from the transposed tables (slots
+  // ks0 + 64 j, field f at kp64[f * kslots + slot]): one contiguous load
+  // per field across the wave (no prefetch of the next pair: its registers
+  // cost these kernels an occupancy step)
+  const int ks0 = (int)b.kpoff[uni_g(pos >> 6)] * 64 + (pos & 63);
+  const double* kt = b.kp64;
+  const long long ns = b.kslots;
+  auto ket = [&](int t, double* y) {
+    y[0] = kt[t];           // eta
+    y[1] = (kt + ns)[t];     // 1/eta
+    y[2] = (kt + 2 * ns)[t]; // ck
+    y[3] = (kt + 3 * ns)[t]; // P
+    y[4] = (kt + 4 * ns)[t];
+    y[5] = (kt + 5 * ns)[t];
+    y[6] = (kt + 6 * ns)[t]; // P - C
+    y[7] = (kt + 7 * ns)[t];
+    y[8] = (kt + 8 * ns)[t];
+    y[9] = (kt + 10 * ns)[t]; // exponent of the C primitive
+  };
   for (int j = 0; j < ncd; ++j) {
-    const PrimPair Y = b.pp[cd0 + j];
-    const double q = Y.eta, c2 = 2.0 * Y.a_exp;
-    const double QC[3] = {Y.Px - cC[0], Y.Py - cC[1], Y.Pz - cC[2]};
+    double y[10];
+    ket(ks0 + 64 * j, y);
+    const double q = y[0], iq = y[1], ckq = y[2], c2 = 2.0 * y[9];
+    const double Yp[3] = {y[3], y[4], y[5]};
+    const double QC[3] = {y[6], y[7], y[8]};
     for (int i = 0; i < nab; ++i) {
       const PrimPair X = b.pp[ab0 + i];  // wave-uniform
       const double p = X.eta, a2 = 2.0 * X.a_exp, b2 = 2.0 * X.b_exp;
       const double spq = p + q, inv_pq = rcp_nr(spq);
-      const double dPQ[3] = {X.Px - Y.Px, X.Py - Y.Py, X.Pz - Y.Pz};
+      const double dPQ[3] = {X.Px - Yp[0], X.Py - Yp[1], X.Pz - Yp[2]};
       const double T = p * q * inv_pq * (dPQ[0] * dPQ[0] + dPQ[1] * dPQ[1] + dPQ[2] * dPQ[2]);
-      const double base = 34.98683665524972497 * X.inv_eta * Y.inv_eta * rsq_nr(spq) * X.cK * Y.cK;
+      // 2 pi^(5/2) cK_ab cK_cd / (p q sqrt(p+q)); the ket's ck folds its share
+      // (PrimPairF::ck = sqrt(2 pi^(5/2)) cK / eta)
+      const double base = kSqrt2Pi52g * X.inv_eta * X.cK * ckq * rsq_nr(spq);
       const double PA[3] = {X.Px - cA[0], X.Py - cA[1], X.Pz - cA[2]};
       double rx[NR], rwt[NR];
       rys_roots_blk<NR, double, (NR <= 3 ? rys_c::lds_stride(NR, 8) : rys_c::interval_size(NR))>(T, tab, rx, rwt);
@@ -210,7 +235,7 @@ __device__ __forceinline__ void deriv_unit(const DevBasis& b, const GradFastArgs
         const double x = rx[u];
         const double B0 = 0.5 * x * inv_pq;
         const double B1 = 0.5 * (1.0 - x) * X.inv_eta + B0;
-        const double B1p = 0.5 * (1.0 - x) * Y.inv_eta + B0;
+        const double B1p = 0.5 * (1.0 - x) * iq + B0;
         const double xq = q * inv_pq * x, xp = p * inv_pq * x;
         double G[3][NM + 1][MM + 1];
 #pragma unroll
@@ -266,15 +291,15 @@ __device__ __forceinline__ void deriv_unit(const DevBasis& b, const GradFastArgs
 
 template <int LA, int LB, int LC, int LD, int NCH, int CH>
 __device__ __forceinline__ void deriv_dispatch(int chunk, const DevBasis& b, const GradFastArgs& g, const double* tab,
-                                               int P, int Qp, bool active, double* sgrad, bool lds_grad) {
+                                               int P, int Qp, int pos, bool active, double* sgrad, bool lds_grad) {
   if constexpr (CH < NCH) {
     constexpr int NCOMP = DPlan<LA, LB, LC, LD>::NCOMP;
     if (chunk == CH) {
-      deriv_unit<LA, LB, LC, LD, CH * NCOMP / NCH, (CH + 1) * NCOMP / NCH>(b, g, tab, P, Qp, active, sgrad,
+      deriv_unit<LA, LB, LC, LD, CH * NCOMP / NCH, (CH + 1) * NCOMP / NCH>(b, g, tab, P, Qp, pos, active, sgrad,
                                                                             lds_grad);
       return;
     }
-    deriv_dispatch<LA, LB, LC, LD, NCH, CH + 1>(chunk, b, g, tab, P, Qp, active, sgrad, lds_grad);
+    deriv_dispatch<LA, LB, LC, LD, NCH, CH + 1>(chunk, b, g, tab, P, Qp, pos, active, sgrad, lds_grad);
   }
 }
 
@@ -312,7 +337,9 @@ __global__ __launch_bounds__(kThreads) void eri_deriv_fast_kernel(DevBasis b, Cl
       const int P = uni_g(un.P);
       const bool active = lane < un.ncol;
       const int Qp = w.col_pair[un.col + (active ? lane : 0)];
-      deriv_dispatch<LA, LB, LC, LD, NCH, 0>(chunk, b, g, tab, P, Qp, active, sgrad, lds_grad);
+      // units start at 64-position blocks, so un.col + lane stays inside the
+      // unit's block (padding positions for the inactive lanes)
+      deriv_dispatch<LA, LB, LC, LD, NCH, 0>(chunk, b, g, tab, P, Qp, un.col + lane, active, sgrad, lds_grad);
     }
     __syncthreads();  // s_task may be overwritten
   }

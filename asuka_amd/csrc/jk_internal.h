@@ -95,9 +95,11 @@ struct DevBasis {
   // The ket primitive pairs by position, transposed so that the lanes of a
   // wave read the j-th primitive pair of their kets at consecutive addresses:
   // block k's primitive j of lane l is slot (kpoff[k] + j) * 64 + l, field f
-  // of slot t at kp64[f * kslots + t] (PrimPairF's ten fields in order; FP64
-  // kernels) and, for the FP32 kernels, kp32d[f * kslots + t] (Px, Py, Pz)
-  // and kp32f[f * kslots + t] (eta, inv_eta, ck, PAx, PAy, PAz, q).
+  // of slot t at kp64[f * kslots + t] (PrimPairF's ten fields in order, then
+  // the exponent of the pair's first primitive (PrimPair::a_exp, for the
+  // derivative kernels); FP64 kernels) and, for the FP32 kernels,
+  // kp32d[f * kslots + t] (Px, Py, Pz) and kp32f[f * kslots + t] (eta,
+  // inv_eta, ck, PAx, PAy, PAz, q).
   const long long* kpoff;
   const double* kp64;
   const double* kp32d;
