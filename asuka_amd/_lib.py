@@ -28,6 +28,7 @@ WANT_K = 2
 MIXED = 4
 COUNT = 8
 TIME = 16
+DETERMINISTIC = 32  # bitwise reproducible sums (64-bit fixed point, ASUKA_JK_DETERMINISTIC)
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)

@@ -222,7 +222,7 @@ __global__ void k_pos_gather(DevBasis b, const int* __restrict__ col, const long
 // of J belongs to one ket position, and no other kernel of the build runs
 // any more, so the adds need no atomics.
 __global__ void k_pos_scatter(DevBasis b, const int* __restrict__ col, const long long* __restrict__ jblk,
-                              const double* __restrict__ Jpos, double* J, double* J2, long long len) {
+                              const double* __restrict__ Jpos, double* J, double* J2, long long len, int fixed) {
   const long long k = blockIdx.x;
   const int lane = threadIdx.x;
   const int sp = col[k * 64 + lane];
@@ -236,19 +236,44 @@ __global__ void k_pos_scatter(DevBasis b, const int* __restrict__ col, const lon
   for (int c = 0; c < ncd; ++c) {
     const long long i = base + 64LL * c + lane;
     const long long e = msh + (aoC + c / nd) * N + aoD + c % nd;
-    if (J) J[e] += Jpos[i];
-    if (J2) J2[e] += Jpos[len + i];
+    if (fixed) {  // 64-bit fixed-point sums (EriOut::fixed)
+      long long* Jf = reinterpret_cast<long long*>(J);
+      long long* J2f = reinterpret_cast<long long*>(J2);
+      const long long* Pf = reinterpret_cast<const long long*>(Jpos);
+      if (J) Jf[e] = (long long)((unsigned long long)Jf[e] + (unsigned long long)Pf[i]);
+      if (J2) J2f[e] = (long long)((unsigned long long)J2f[e] + (unsigned long long)Pf[len + i]);
+    } else {
+      if (J) J[e] += Jpos[i];
+      if (J2) J2[e] += Jpos[len + i];
+    }
   }
 }
 
 // X[i][j] += Xc[i][g(j)]: the permuted-column flush buffer (EriOut::Kc) into
 // the output, g(j) the compact column of AO j.
-__global__ void k_unperm_add(double* X, const double* __restrict__ Xc, const int* __restrict__ ao_g, int n) {
+__global__ void k_unperm_add(double* X, const double* __restrict__ Xc, const int* __restrict__ ao_g, int n,
+                             int fixed) {
   const long long nn = (long long)n * n;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nn; t += (long long)gridDim.x * blockDim.x) {
     const long long i = t / n, j = t - i * n;
-    X[t] += Xc[i * n + ao_g[j]];
+    if (fixed) {  // 64-bit fixed-point sums (EriOut::fixed)
+      unsigned long long* Xf = reinterpret_cast<unsigned long long*>(X);
+      Xf[t] += reinterpret_cast<const unsigned long long*>(Xc)[i * n + ao_g[j]];
+    } else {
+      X[t] += Xc[i * n + ao_g[j]];
+    }
   }
+}
+
+// Deterministic builds: X (n doubles) to the 64-bit fixed point of the build
+// (units of 2^-44, stored in place) and back.
+__global__ void k_to_fixed(double* X, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    X[t] = __longlong_as_double(__double2ll_rn(X[t] * kFixScale));
+}
+__global__ void k_from_fixed(double* X, long long n) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+    X[t] = (double)__double_as_longlong(X[t]) * (1.0 / kFixScale);
 }
 
 // out = (X + X^T) / 2 for an n x n block X (batch builds: the packed block of
@@ -1550,14 +1575,16 @@ static int run_classes(asuka_jk_ctx* ctx, const EriOut& o, hipStream_t s, bool t
   if ((rc = join_streams(ctx, s))) return rc;
   if (pos) {
     hipLaunchKernelGGL(k_pos_scatter, dim3((unsigned)ctx->nblk), dim3(64), 0, s, *dev, ctx->d_col_pair, ctx->d_jblk,
-                       ctx->d_jpos, Jt, Jt2, ctx->pos_len);
+                       ctx->d_jpos, Jt, Jt2, ctx->pos_len, o.fixed);
     HIP_CHECK(hipGetLastError());
   }
   if (perm) {
-    if (Kt) hipLaunchKernelGGL(k_unperm_add, dim3(grid_for(nn, 256)), dim3(256), 0, s, Kt, ctx->d_kc, ctx->d_ao_g, ctx->nao);
+    if (Kt)
+      hipLaunchKernelGGL(k_unperm_add, dim3(grid_for(nn, 256)), dim3(256), 0, s, Kt, ctx->d_kc, ctx->d_ao_g, ctx->nao,
+                         o.fixed);
     if (Kt2)
       hipLaunchKernelGGL(k_unperm_add, dim3(grid_for(nn, 256)), dim3(256), 0, s, Kt2, ctx->d_kc + nn, ctx->d_ao_g,
-                         ctx->nao);
+                         ctx->nao, o.fixed);
     HIP_CHECK(hipGetLastError());
   }
   if (timed) HIP_CHECK(hipEventRecord(ctx->ev_stop, s));
@@ -1596,7 +1623,8 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
   }
   o.eps_density = ctx->eps_density;
   o.eps_prim = ctx->eps_schwarz > 0.0 ? ctx->eps_prim : 0.0;
-  o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
+  o.fixed = (flags & ASUKA_JK_DETERMINISTIC) ? 1 : 0;  // the zeroed outputs are fixed-point zeros too
+  o.mixed = ((flags & ASUKA_JK_MIXED) && !o.fixed) ? 1 : 0;  // deterministic builds run every quartet in FP64
   o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
     const long long nc = kCountStride * (long long)std::max<size_t>(1, ctx->cps.size());
@@ -1605,6 +1633,11 @@ static int jk_build_impl(asuka_jk_ctx* ctx, const double* D, const double* D2, d
   }
   int rc = run_classes(ctx, o, s, (flags & ASUKA_JK_TIME) != 0, queue, &dev);
   if (rc) return rc;
+  if (o.fixed) {
+    for (double* X : outs)
+      if (X) hipLaunchKernelGGL(k_from_fixed, dim3(grid_for(nn, 256)), dim3(256), 0, s, X, nn);
+    HIP_CHECK(hipGetLastError());
+  }
   if (!symmetrize) return ASUKA_OK;
   const int n = ctx->nao;
   dim3 blk(16, 16), grd((n + 15) / 16, (n + 15) / 16);
@@ -1623,6 +1656,7 @@ extern "C" {
 int asuka_jk_batch_build(asuka_jk_ctx* ctx, const double* const* D, double* const* J, double* const* K, int flags,
                          void* stream) {
   if (!ctx || !D) return fail(ASUKA_E_INVALID, "ctx and D must be non-null");
+  if (flags & ASUKA_JK_DETERMINISTIC) return fail(ASUKA_E_INVALID, "batch builds do not take ASUKA_JK_DETERMINISTIC");
   const int nmol = ctx->nmol;
   for (int m = 0; m < nmol; ++m)
     if (!D[m]) return fail(ASUKA_E_INVALID, "every D[m] must be non-null");
@@ -1738,16 +1772,19 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   hipStream_t s = (hipStream_t)stream;
   const long long nn = (long long)ctx->nao * ctx->nao;
   if (F != hcore) hipLaunchKernelGGL(k_copy, dim3(grid_for(nn, 256)), dim3(256), 0, s, F, hcore, nn);
+  const bool fixed = (flags & ASUKA_JK_DETERMINISTIC) != 0;
+  if (fixed) hipLaunchKernelGGL(k_to_fixed, dim3(grid_for(nn, 256)), dim3(256), 0, s, F, nn);
   hipLaunchKernelGGL(k_dsp, dim3(grid_for(ctx->nsp, 128)), dim3(128), 0, s, D, nullptr, ctx->dev, ctx->d_Dsp,
                      ctx->d_kd, ctx->d_pos_of_sp);
   HIP_CHECK(hipGetLastError());
   EriOut o{};
   o.mode = MODE_FOCK;
+  o.fixed = fixed ? 1 : 0;
   o.D = D;
   o.F = F;
   o.eps_density = ctx->eps_density;
   o.eps_prim = ctx->eps_schwarz > 0.0 ? ctx->eps_prim : 0.0;
-  o.mixed = (flags & ASUKA_JK_MIXED) ? 1 : 0;
+  o.mixed = ((flags & ASUKA_JK_MIXED) && !fixed) ? 1 : 0;  // deterministic builds run every quartet in FP64
   o.tau_mixed = ctx->tau_mixed;
   if (flags & ASUKA_JK_COUNT) {
     const long long nc = kCountStride * (long long)std::max<size_t>(1, ctx->cps.size());
@@ -1756,6 +1793,7 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
   }
   int rc = run_classes(ctx, o, s, (flags & ASUKA_JK_TIME) != 0);
   if (rc) return rc;
+  if (fixed) hipLaunchKernelGGL(k_from_fixed, dim3(grid_for(nn, 256)), dim3(256), 0, s, F, nn);
   const int n = ctx->nao;
   dim3 blk(16, 16), grd((n + 15) / 16, (n + 15) / 16);
   hipLaunchKernelGGL(k_symmetrize, grd, blk, 0, s, F, n);
@@ -1766,6 +1804,7 @@ int asuka_fock_build(asuka_jk_ctx* ctx, const double* D, const double* hcore, do
 int asuka_jk_build_batched(int nmol, asuka_jk_ctx* const* ctxs, const double* const* D, double* const* J,
                            double* const* K, int flags, void* stream) {
   if (nmol < 0) return fail(ASUKA_E_INVALID, "nmol must be >= 0");
+  if (flags & ASUKA_JK_DETERMINISTIC) return fail(ASUKA_E_INVALID, "batch builds do not take ASUKA_JK_DETERMINISTIC");
   if (nmol > 0 && (!ctxs || !D)) return fail(ASUKA_E_INVALID, "ctxs and D must be non-null");
   for (int m = 0; m < nmol; ++m)
     if (!ctxs[m] || !D[m]) return fail(ASUKA_E_INVALID, "every ctxs[m] and D[m] must be non-null");

@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (!Jo) continue;
             double s = 0.0;
             for (int icd = 0; icd < NCD; ++icd) s += tile[r * NCD + icd] * Dcd[icd];
-            atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], fj * s);
+            acc_add(&Jo[(aoA + r / NB) * N + aoB + r % NB], fj * s, o.fixed);
             continue;
           }
           r -= NAB;
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (!Jo || !bk) continue;
             double s = 0.0;
             for (int iab = 0; iab < NAB; ++iab) s += tile[iab * NCD + r] * Dab[iab];
-            atomicAdd(&Jo[(aoC + r / ND) * N + aoD + r % ND], fj * s);
+            acc_add(&Jo[(aoC + r / ND) * N + aoD + r % ND], fj * s, o.fixed);
             continue;
           }
           r -= NCD;
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             double s = 0.0;
             for (int ib = 0; ib < NB; ++ib)
               for (int id = 0; id < ND; ++id) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dbd[ib * ND + id];
-            atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], fk * s);
+            acc_add(&Ko[(aoA + ia) * N + aoC + ic], fk * s, o.fixed);
             continue;
           }
           r -= NA * NC;
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             double s = 0.0;
             for (int ib = 0; ib < NB; ++ib)
               for (int ic = 0; ic < NC; ++ic) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dbc[ib * NC + ic];
-            atomicAdd(&Ko[(aoA + ia) * N + aoD + id], fk * s);
+            acc_add(&Ko[(aoA + ia) * N + aoD + id], fk * s, o.fixed);
             continue;
           }
           r -= NA * ND;
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             double s = 0.0;
             for (int ia = 0; ia < NA; ++ia)
               for (int id = 0; id < ND; ++id) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dad[ia * ND + id];
-            atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], fk * s);
+            acc_add(&Ko[(aoB + ib) * N + aoC + ic], fk * s, o.fixed);
             continue;
           }
           r -= NB * NC;
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             double s = 0.0;
             for (int ia = 0; ia < NA; ++ia)
               for (int ic = 0; ic < NC; ++ic) s += tile[((ia * NB + ib) * NC + ic) * ND + id] * Dac[ia * NC + ic];
-            atomicAdd(&Ko[(aoB + ib) * N + aoD + id], fk * s);
+            acc_add(&Ko[(aoB + ib) * N + aoD + id], fk * s, o.fixed);
           }
         }
       }  // densities

@@ -272,7 +272,7 @@ struct RowAcc {
 // jb: the unit's block in the ket-position buffers (EriOut::jblk; uniform),
 // or -1 for D[c,d] from D and J[c,d] atomics on J.
 
-template <class PL, int CH, typename CT>
+template <class PL, int CH, typename CT, bool FX>
 __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, int P, int Qp, int pos, int nab,
                                           bool active, const CT* __restrict__ tab, const RowAcc& acc, double* jcd,
                                           long long jb) {
@@ -495,9 +495,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
         s = wave_sum(active ? fj * s : 0.0);
         if (lane == first && active) {
           if (lds)
-            atomicAdd(&s_acc[so + acc.jab + (a0 + r / NB) * NB + r % NB], s);
+            acc_add(&s_acc[so + acc.jab + (a0 + r / NB) * NB + r % NB], s, FX);
           else
-            atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], s);
+            acc_add(&Jo[(aoA + r / NB) * N + aoB + r % NB], s, FX);
         }
       }
       if (bk && active) {
@@ -514,9 +514,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
           if (jcd && di == 0)
             jcd[c] = fma(fj, s, jcd[c]);
           else if (jb >= 0)
-            atomicAdd((di ? o.Jpos2 : o.Jpos) + jb + 64LL * (c0 * ND + c) + lane, fj * s);
+            acc_add((di ? o.Jpos2 : o.Jpos) + jb + 64LL * (c0 * ND + c) + lane, fj * s, FX);
           else
-            atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s);
+            acc_add(&Jo[(aoC + c / ND) * N + aoD + c % ND], fj * s, FX);
         }
       }
     }
@@ -537,9 +537,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
               for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbd[ib * ND + id], s);
             if (lds)
-              atomicAdd(&s_acc[so + acc.kac + (a0 + ia) * acc.colsC + colC + ic], ks * s);
+              acc_add(&s_acc[so + acc.kac + (a0 + ia) * acc.colsC + colC + ic], ks * s, FX);
             else
-              atomicAdd(&Ko[(aoA + ia) * N + aoC + ic], ks * s);
+              acc_add(&Ko[(aoA + ia) * N + aoC + ic], ks * s, FX);
           }
       }
       if (cd_neq) {  // K[a,d] += sum_{b,c} v D[b,c]
@@ -558,9 +558,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
               for (int ic = 0; ic < NCL; ++ic) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dbc[ib * NCL + ic], s);
             if (lds)
-              atomicAdd(&s_acc[so + acc.kad + (a0 + ia) * acc.colsD + colD + id], ks * s);
+              acc_add(&s_acc[so + acc.kad + (a0 + ia) * acc.colsD + colD + id], ks * s, FX);
             else
-              atomicAdd(&Ko[(aoA + ia) * N + aoD + id], ks * s);
+              acc_add(&Ko[(aoA + ia) * N + aoD + id], ks * s, FX);
           }
       }
       if (ab_neq) {  // K[b,c] += sum_{a,d} v D[a,d]
@@ -579,9 +579,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
 #pragma unroll
               for (int id = 0; id < ND; ++id) s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dad[ia * ND + id], s);
             if (lds)
-              atomicAdd(&s_acc[so + acc.kbc + ib * acc.colsC + colC + ic], ks * s);
+              acc_add(&s_acc[so + acc.kbc + ib * acc.colsC + colC + ic], ks * s, FX);
             else
-              atomicAdd(&Ko[(aoB + ib) * N + aoC + ic], ks * s);
+              acc_add(&Ko[(aoB + ib) * N + aoC + ic], ks * s, FX);
           }
         if (cd_neq) {  // K[b,d] += sum_{a,c} v D[a,c]
           double dac[NAL * NCL];
@@ -600,9 +600,9 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
                 for (int ic = 0; ic < NCL; ++ic)
                   s = fma((double)tile[(ia * NB + ib) * NCDL + ic * ND + id], dac[ia * NCL + ic], s);
               if (lds)
-                atomicAdd(&s_acc[so + acc.kbd + ib * acc.colsD + colD + id], ks * s);
+                acc_add(&s_acc[so + acc.kbd + ib * acc.colsD + colD + id], ks * s, FX);
               else
-                atomicAdd(&Ko[(aoB + ib) * N + aoD + id], ks * s);
+                acc_add(&Ko[(aoB + ib) * N + aoD + id], ks * s, FX);
             }
         }
       }
@@ -610,34 +610,34 @@ __device__ __forceinline__ void run_chunk(const DevBasis& b, const EriOut& o, in
   }  // densities
 }
 
-template <class PL, int CH, typename CT>
+template <class PL, int CH, typename CT, bool FX>
 __device__ __forceinline__ void dispatch_chunk(int chunk, const DevBasis& b, const EriOut& o, int P, int Qp, int pos,
                                                int nab, bool active, const CT* __restrict__ tab, const RowAcc& acc,
                                                long long jb) {
   if constexpr (CH < PL::NCH) {
     if (chunk == CH) {
-      run_chunk<PL, CH, CT>(b, o, P, Qp, pos, nab, active, tab, acc, nullptr, jb);
+      run_chunk<PL, CH, CT, FX>(b, o, P, Qp, pos, nab, active, tab, acc, nullptr, jb);
       return;
     }
-    dispatch_chunk<PL, CH + 1, CT>(chunk, b, o, P, Qp, pos, nab, active, tab, acc, jb);
+    dispatch_chunk<PL, CH + 1, CT, FX>(chunk, b, o, P, Qp, pos, nab, active, tab, acc, jb);
   }
 }
 
 // The A chunks CA.. of C chunk CC, in sequence, carrying J[c,d] in jcd.
-template <class PL, int CC, int CA, typename CT>
+template <class PL, int CC, int CA, typename CT, bool FX>
 __device__ __forceinline__ void run_agroup(const DevBasis& b, const EriOut& o, int P, int Qp, int pos, int nab,
                                            bool active, const CT* __restrict__ tab, const RowAcc& acc, double* jcd,
                                            long long jb) {
   if constexpr (CA < PL::SPLIT_A) {
-    run_chunk<PL, CA * PL::SPLIT_C + CC, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
-    run_agroup<PL, CC, CA + 1, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
+    run_chunk<PL, CA * PL::SPLIT_C + CC, CT, FX>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
+    run_agroup<PL, CC, CA + 1, CT, FX>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
   }
 }
 
 // One wave item of a CARRY plan: C chunk CC of a unit, all of its A chunks,
 // with the J[c,d] sums carried across the A chunks in registers, so J[c,d]
 // receives one atomic per element and quartet instead of one per A chunk.
-template <class PL, int CC, typename CT>
+template <class PL, int CC, typename CT, bool FX>
 __device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, int P, int Qp, int pos, int nab,
                                           bool active, const CT* __restrict__ tab, const RowAcc& acc, long long jb) {
   using CK = Chunk<PL, CC>;  // A chunk 0: the C range of every A chunk of CC
@@ -645,12 +645,12 @@ __device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, in
   double jcd[NCDL];
 #pragma unroll
   for (int c = 0; c < NCDL; ++c) jcd[c] = 0.0;
-  run_agroup<PL, CC, 0, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
+  run_agroup<PL, CC, 0, CT, FX>(b, o, P, Qp, pos, nab, active, tab, acc, jcd, jb);
   double* Jo = (o.mode == MODE_FOCK) ? o.F : o.J;
   if (Jo && active && P != Qp && jb >= 0) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int c = 0; c < NCDL; ++c) atomicAdd(o.Jpos + jb + 64LL * (CK::c0 * ND + c) + lane, jcd[c]);
+    for (int c = 0; c < NCDL; ++c) acc_add(o.Jpos + jb + 64LL * (CK::c0 * ND + c) + lane, jcd[c], FX);
   } else if (Jo && active && P != Qp) {
     const int A = uni(b.sp_A[P]), C = b.sp_A[Qp], Dsh = b.sp_B[Qp];
     long long N, msh;
@@ -658,24 +658,24 @@ __device__ __forceinline__ void run_citem(const DevBasis& b, const EriOut& o, in
     Jo = shifted(Jo, msh);
     const int aoC = b.shell_ao[C] + CK::c0, aoD = b.shell_ao[Dsh];
 #pragma unroll
-    for (int c = 0; c < NCDL; ++c) atomicAdd(&Jo[(aoC + c / ND) * N + aoD + c % ND], jcd[c]);
+    for (int c = 0; c < NCDL; ++c) acc_add(&Jo[(aoC + c / ND) * N + aoD + c % ND], jcd[c], FX);
   }
 }
 
 // Wave item `item` of a unit: a C chunk with its A chunks for CARRY plans,
 // one chunk otherwise.
-template <class PL, int CC, typename CT>
+template <class PL, int CC, typename CT, bool FX>
 __device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const EriOut& o, int P, int Qp, int pos,
                                                int nab, bool active, const CT* __restrict__ tab, const RowAcc& acc,
                                                long long jb) {
   if constexpr (!PL::CARRY) {
-    dispatch_chunk<PL, 0, CT>(cc, b, o, P, Qp, pos, nab, active, tab, acc, jb);
+    dispatch_chunk<PL, 0, CT, FX>(cc, b, o, P, Qp, pos, nab, active, tab, acc, jb);
   } else if constexpr (CC < PL::SPLIT_C) {
     if (cc == CC) {
-      run_citem<PL, CC, CT>(b, o, P, Qp, pos, nab, active, tab, acc, jb);
+      run_citem<PL, CC, CT, FX>(b, o, P, Qp, pos, nab, active, tab, acc, jb);
       return;
     }
-    dispatch_citem<PL, CC + 1, CT>(cc, b, o, P, Qp, pos, nab, active, tab, acc, jb);
+    dispatch_citem<PL, CC + 1, CT, FX>(cc, b, o, P, Qp, pos, nab, active, tab, acc, jb);
   }
 }
 
@@ -698,7 +698,10 @@ __device__ __forceinline__ void dispatch_citem(int cc, const DevBasis& b, const 
 // into b, w and o: with no possible aliasing store (the J/K atomics) the
 // compiler can turn wave-uniform loads (wave unit, bra pair, bra primitive
 // pairs, D blocks of the bra) into scalar loads.
-template <class PL, typename CT>
+// FX: accumulate in the 64-bit fixed point of deterministic builds
+// (EriOut::fixed), a separate instantiation so that the FP64-atomic kernels
+// carry none of its code or registers.
+template <class PL, typename CT, bool FX>
 __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, ClassWork w, EriOut o,
                                                        const PrimPairF* __restrict__ ppf,
                                                        const PrimPairF32* __restrict__ ppf32,
@@ -853,7 +856,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
       const int nab = uni(o.eps_prim > 0.0 ? un.nab : b.pp_count[P]);
       // the unit's block of the ket-position buffers (units start at block boundaries)
       const long long jb = o.jblk ? o.jblk[un.col >> 6] : -1LL;
-      dispatch_citem<PL, 0, CT>(chunk, b, o, P, Qp, pos, nab, active, tab, acc, jb);
+      dispatch_citem<PL, 0, CT, FX>(chunk, b, o, P, Qp, pos, nab, active, tab, acc, jb);
     }
     if (use_acc) {
       if (worked && lane == 0) s_work = 1;
@@ -871,8 +874,8 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
         // compact columns of the C and D lists (EriOut::Kc)
         const int gC = w.lao_offC + mlao[PL::lc], gD = cD > 0 ? w.lao_offD + mlao[PL::ld] : gC;
         for (int t2 = threadIdx.x; t2 < nacc * nden; t2 += nthr) {
-          const double v = s_acc[t2];
-          if (v == 0.0) continue;
+          const double v = s_acc[t2];  // fixed builds: the bits of an integer sum
+          if (__double_as_longlong(v) == 0LL) continue;
           s_acc[t2] = 0.0;
           const bool second = t2 >= nacc;
           double* Jo = shifted(second ? o.J2 : ((o.mode == MODE_FOCK) ? o.F : o.J), msh);
@@ -880,7 +883,7 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
           double* Kc = second ? o.Kc2 : o.Kc;  // uniform
           int r = second ? t2 - nacc : t2;
           if (r < NA * NB) {
-            atomicAdd(&Jo[(aoA + r / NB) * N + aoB + r % NB], v);
+            acc_add_sum(&Jo[(aoA + r / NB) * N + aoB + r % NB], v, FX);
             continue;
           }
           r -= NA * NB;
@@ -901,9 +904,9 @@ __global__ __launch_bounds__(kGroupThreads) void eri_fast_kernel(DevBasis b, Cla
             }
           }
           if (Kc)  // consecutive entries, consecutive addresses
-            atomicAdd(&Kc[row * N + g + k], v);
+            acc_add_sum(&Kc[row * N + g + k], v, FX);
           else
-            atomicAdd(&Ko[row * N + lao[k]], v);
+            acc_add_sum(&Ko[row * N + lao[k]], v, FX);
         }
       }
     }
@@ -936,14 +939,14 @@ struct F32PlanOf {
   using type = Plan<PL::la, PL::lb, PL::lc, PL::ld, SA, SC>;
 };
 
-template <class PL, typename CT>
+template <class PL, typename CT, bool FX = false>
 hipError_t launch_one(const DevBasis& b, const ClassWork& w, const EriOut& o, size_t lds, long long g, hipStream_t s) {
   if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute((const void*)eri_fast_kernel<PL, CT>,
+    const hipError_t e = hipFuncSetAttribute((const void*)eri_fast_kernel<PL, CT, FX>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((eri_fast_kernel<PL, CT>), dim3((unsigned)g), dim3(64 * w.wpg), lds, s, b, w, o, b.ppf, b.ppf32,
+  hipLaunchKernelGGL((eri_fast_kernel<PL, CT, FX>), dim3((unsigned)g), dim3(64 * w.wpg), lds, s, b, w, o, b.ppf, b.ppf32,
                      w.units, w.col_pair, b.sp_A, b.sp_B, b.pp_start, b.pp_count, b.shell_xyz, b.shell_ao, b.Q,
                      b.Dsp, o.D);
   return hipGetLastError();
@@ -959,6 +962,8 @@ hipError_t launch_plan(const DevBasis& b, const ClassWork& w, const EriOut& o, h
   if (g < 1) g = 1;
   const size_t lds =
       w.acc_colsC > 0 ? (o.D2 ? 2 : 1) * row_acc_bytes(PL::la, PL::lb, w.acc_colsC, w.acc_colsD) : 0;
+  // deterministic builds (no mixed routing: every unit on the FP64 kernel)
+  if (o.fixed) return launch_one<PL, double, true>(b, w, o, lds, g, s);
   hipError_t e = launch_one<PL, double>(b, w, o, lds, g, s);
   if (e != hipSuccess || !o.mixed) return e;
   ClassWork w32 = w;

@@ -255,15 +255,19 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
           continue;
         }
         const int ia = iab / nB, ib = iab - ia * nB, ic = icd / nD, id = icd - ic * nD;
+        // deterministic builds: the block sums are fixed point with the J and K
+        // factors applied here, so the flush adds them unchanged
+        const double vj = o.fixed ? v * ((ab_neq ? 2.0 : 1.0) * (cd_neq ? 2.0 : 1.0)) : v;
+        const double vk = o.fixed ? v * ((bk ? 2.0 : 1.0) * (o.mode == MODE_FOCK ? -0.5 : 1.0)) : v;
         for (int di = 0; di < nden; ++di) {
           const int so = di * 2 * nblk;
-          atomicAdd(&Lj_ab[so + iab], v * Dcd[so + icd]);
-          if (bk) atomicAdd(&Lj_cd[so + icd], v * Dab[so + iab]);
-          atomicAdd(&Lk_ac[so + ia * nC + ic], v * Dbd[so + ib * nD + id]);
-          if (cd_neq) atomicAdd(&Lk_ad[so + ia * nD + id], v * Dbc[so + ib * nC + ic]);
+          acc_add(&Lj_ab[so + iab], vj * Dcd[so + icd], o.fixed);
+          if (bk) acc_add(&Lj_cd[so + icd], vj * Dab[so + iab], o.fixed);
+          acc_add(&Lk_ac[so + ia * nC + ic], vk * Dbd[so + ib * nD + id], o.fixed);
+          if (cd_neq) acc_add(&Lk_ad[so + ia * nD + id], vk * Dbc[so + ib * nC + ic], o.fixed);
           if (ab_neq) {
-            atomicAdd(&Lk_bc[so + ib * nC + ic], v * Dad[so + ia * nD + id]);
-            if (cd_neq) atomicAdd(&Lk_bd[so + ib * nD + id], v * Dac[so + ia * nC + ic]);
+            acc_add(&Lk_bc[so + ib * nC + ic], vk * Dad[so + ia * nD + id], o.fixed);
+            if (cd_neq) acc_add(&Lk_bd[so + ib * nD + id], vk * Dac[so + ia * nC + ic], o.fixed);
           }
         }
       }
@@ -287,22 +291,27 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         const double* Lkad = Lkac + nA * nC;
         const double* Lkbc = Lkad + nA * nD;
         const double* Lkbd = Lkbc + nB * nC;
+        // fixed: the block sums already carry fj / fk (acc_add_sum adds them as they are)
+        const double sj = o.fixed ? 1.0 : fj, sk = o.fixed ? 1.0 : fk;
+        auto add = [&](double* p, double sum, double sc) {
+          if (o.fixed)
+            acc_add_sum(p, sum, true);
+          else
+            atomicAdd(p, sc * sum);
+        };
         if (Jo) {
-          for (int e = lane; e < nAB; e += kWave) atomicAdd(&Jo[(aoA + e / nB) * N + aoB + e % nB], fj * Ljab[e]);
+          for (int e = lane; e < nAB; e += kWave) add(&Jo[(aoA + e / nB) * N + aoB + e % nB], Ljab[e], sj);
           if (bk)
-            for (int e = lane; e < nCD; e += kWave) atomicAdd(&Jo[(aoC + e / nD) * N + aoD + e % nD], fj * Ljcd[e]);
+            for (int e = lane; e < nCD; e += kWave) add(&Jo[(aoC + e / nD) * N + aoD + e % nD], Ljcd[e], sj);
         }
         if (Ko) {
-          for (int e = lane; e < nA * nC; e += kWave) atomicAdd(&Ko[(aoA + e / nC) * N + aoC + e % nC], fk * Lkac[e]);
+          for (int e = lane; e < nA * nC; e += kWave) add(&Ko[(aoA + e / nC) * N + aoC + e % nC], Lkac[e], sk);
           if (cd_neq)
-            for (int e = lane; e < nA * nD; e += kWave)
-              atomicAdd(&Ko[(aoA + e / nD) * N + aoD + e % nD], fk * Lkad[e]);
+            for (int e = lane; e < nA * nD; e += kWave) add(&Ko[(aoA + e / nD) * N + aoD + e % nD], Lkad[e], sk);
           if (ab_neq) {
-            for (int e = lane; e < nB * nC; e += kWave)
-              atomicAdd(&Ko[(aoB + e / nC) * N + aoC + e % nC], fk * Lkbc[e]);
+            for (int e = lane; e < nB * nC; e += kWave) add(&Ko[(aoB + e / nC) * N + aoC + e % nC], Lkbc[e], sk);
             if (cd_neq)
-              for (int e = lane; e < nB * nD; e += kWave)
-                atomicAdd(&Ko[(aoB + e / nD) * N + aoD + e % nD], fk * Lkbd[e]);
+              for (int e = lane; e < nB * nD; e += kWave) add(&Ko[(aoB + e / nD) * N + aoD + e % nD], Lkbd[e], sk);
           }
         }
       }

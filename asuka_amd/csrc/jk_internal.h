@@ -222,7 +222,33 @@ struct EriOut {
   // launches. Kc2: the second density's.
   double* Kc;
   double* Kc2;
+  // ASUKA_JK_DETERMINISTIC: J, K, F, Jpos and Kc hold 64-bit two's-complement
+  // fixed point (units of 2^-44) during the build and every contribution is
+  // added with an integer atomic (acc_add), so the result does not depend on
+  // the order of the adds
+  int fixed;
 };
+
+// Fixed-point unit of deterministic builds: 2^44 per unit value (resolution
+// 5.7e-14, range +-2^19 for the 64-bit sums).
+constexpr double kFixScale = 17592186044416.0;
+
+// p += v: an FP64 atomic add, or with fixed set an integer atomic add of v in
+// units of 2^-44 (p then holds the 64-bit fixed-point sum).
+__device__ __forceinline__ void acc_add(double* p, double v, bool fixed) {
+  if (fixed)
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double2ll_rn(v * kFixScale));
+  else
+    atomicAdd(p, v);
+}
+// p += a partial sum read from an accumulator of the same representation
+// (fixed: the 64-bit pattern of an integer sum).
+__device__ __forceinline__ void acc_add_sum(double* p, double v, bool fixed) {
+  if (fixed)
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+  else
+    atomicAdd(p, v);
+}
 
 constexpr int kCountStride = 16;
 

@@ -271,6 +271,7 @@ def direct_JK(
     mixed_precision: bool | None = None,
     f32_accum: bool | None = None,
     out: tuple | None = None,
+    deterministic: bool = False,
 ):
     """J and K for density D (asuka/hf/direct_jk.py:1530-2202).
 
@@ -278,6 +279,8 @@ def direct_JK(
     for an output not wanted). ``out=(J, K)`` reuses caller buffers.
     ``tile_dtype`` and ``f32_accum`` are accepted for signature compatibility;
     this engine never materialises tiles and accumulates in FP64.
+    ``deterministic``: bitwise reproducible J and K (ASUKA_JK_DETERMINISTIC:
+    contributions rounded to 2^-44 and summed as 64-bit integers).
     """
     del tile_dtype, f32_accum
     if not want_J and not want_K:
@@ -293,6 +296,8 @@ def direct_JK(
     flags = (_lib.WANT_J if want_J else 0) | (_lib.WANT_K if want_K else 0) | _flags(mixed_precision)
     if stats is not None:
         flags |= _lib.COUNT
+    if deterministic:
+        flags |= _lib.DETERMINISTIC
     t0 = time.perf_counter()
     _lib.check(_lib.lib().asuka_jk_build(ctypes.c_void_p(ctx.handle), _lib.ptr(Dd), _lib.ptr(J if want_J else None),
                                          _lib.ptr(K if want_K else None), flags,
@@ -329,6 +334,7 @@ def direct_JK_multi(
     mixed_precision: bool | None = None,
     f32_accum: bool | None = None,
     out: tuple | None = None,
+    deterministic: bool = False,
 ):
     """(Ja, Ka, Jb, Kb) for two densities, each ERI evaluated once
     (asuka/hf/direct_jk.py:2716-3095).
@@ -337,7 +343,8 @@ def direct_JK_multi(
     (Ja, Ka) and with Db into (Jb, Kb) in the same kernel, so the cost is one
     ERI pass instead of two. Returns symmetrised (nao, nao) float64 GPU
     tensors (None for outputs not wanted); ``out=(Ja, Ka, Jb, Kb)`` reuses
-    caller buffers.
+    caller buffers. ``deterministic``: bitwise reproducible outputs (see
+    :func:`direct_JK`).
     """
     del tile_dtype, f32_accum
     if not want_J and not want_K:
@@ -360,6 +367,8 @@ def direct_JK_multi(
     if not want_K:
         Ka = Kb = None
     flags = (_lib.WANT_J if want_J else 0) | (_lib.WANT_K if want_K else 0) | _flags(mixed_precision)
+    if deterministic:
+        flags |= _lib.DETERMINISTIC
     t0 = time.perf_counter()
     _lib.check(_lib.lib().asuka_jk_build_multi(ctypes.c_void_p(ctx.handle), _lib.ptr(Dad), _lib.ptr(Dbd),
                                                _lib.ptr(Ja), _lib.ptr(Ka), _lib.ptr(Jb), _lib.ptr(Kb), flags,
@@ -385,8 +394,10 @@ def direct_fock_rhf(
     mixed_precision: bool | None = None,
     f32_accum: bool | None = None,
     out=None,
+    deterministic: bool = False,
 ):
-    """F = h + J(D) - K(D)/2, symmetrised (asuka/hf/direct_jk.py:2205-2713)."""
+    """F = h + J(D) - K(D)/2, symmetrised (asuka/hf/direct_jk.py:2205-2713).
+    ``deterministic``: bitwise reproducible F (see :func:`direct_JK`)."""
     del tile_dtype, f32_accum
     torch = _torch()
     nao = ctx.nao
@@ -395,6 +406,8 @@ def direct_fock_rhf(
     F = out if out is not None else torch.empty((nao, nao), dtype=torch.float64, device=ctx.device)
     t0 = time.perf_counter()
     flags = _flags(mixed_precision) | (_lib.COUNT if stats is not None else 0)
+    if deterministic:
+        flags |= _lib.DETERMINISTIC
     _lib.check(_lib.lib().asuka_fock_build(ctypes.c_void_p(ctx.handle), _lib.ptr(Dd), _lib.ptr(hd), _lib.ptr(F),
                                            flags, ctypes.c_void_p(_lib.stream_handle(ctx.device))))
     if profile is not None:

@@ -48,6 +48,12 @@ extern "C" {
 #define ASUKA_JK_MIXED 4 /* per-quartet FP32 recursion below opts.tau_mixed */
 #define ASUKA_JK_COUNT 8 /* also count the quartets evaluated (asuka_jk_last_counts) */
 #define ASUKA_JK_TIME 16 /* record HIP events around the ERI kernels (asuka_jk_last_timing) */
+#define ASUKA_JK_DETERMINISTIC 32 /* bitwise reproducible outputs: every J/K/F contribution is rounded to a
+                                     multiple of 2^-44 and summed as a 64-bit integer (order-independent),
+                                     converted to FP64 once at the end; |element| must stay below 2^19.
+                                     Every quartet runs in FP64 (ASUKA_JK_MIXED is ignored). asuka_jk_build,
+                                     asuka_jk_build_noalloc, asuka_jk_build_multi and asuka_fock_build; the
+                                     batch builds reject it */
 
 typedef struct asuka_jk_ctx asuka_jk_ctx;
 
