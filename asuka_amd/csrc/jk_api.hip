@@ -36,8 +36,11 @@ static thread_local std::string g_err;
 // default FP32 threshold on Q_ab Q_cd max|D| for builds with ASUKA_JK_MIXED
 static constexpr double kDefaultTauMixed = 1e-5;
 // Row tasks: at most this many wave units of one bra row per workgroup.
-static constexpr int kTaskUnits = 32;
-static constexpr int kTaskItems = 256;  // wave items (unit x chunk) per row task, for split plans
+// Long tasks flush the LDS row accumulators less often: measured at (H2O)64
+// cc-pVDZ, at least 128 units per task took the FP64 Fock build from 476 to
+// 447 ms and the mixed one from 396 to 332 ms against 16-32, and left (H2O)8
+// cc-pVTZ unchanged (a task never spans two bra rows, so short rows stay short).
+static constexpr int kTaskUnits = 128;
 // LDS accumulators of one row task larger than this (or than the device allows
 // next to the root tables) fall back to global atomics; ASUKA_ROW_ACC_MAX_KB
 // lowers it. gfx950 gives a workgroup up to 160 KB: accumulators above 80 KB
@@ -1150,12 +1153,6 @@ static int ctx_build(const asuka_basis_desc* bd, const std::vector<int>* shell_m
         cp.acc_colsC = fits ? cC : 0;
         cp.acc_colsD = fits ? cD : 0;
         cp.wpg = (fits && bytes > kRowAccSmallBytes) ? 8 : 4;
-        task_units = cp.wpg == 8 ? kTaskUnits : kTaskUnits / 2;
-        // a unit of a split plan is nch wave items: keep a row task at about
-        // kTaskItems items, so classes with many chunks get short tasks (a
-        // short tail, the queue balances them)
-        const int nch = eri_fast_nch(cp.la, cp.lb, cp.lc, cp.ld);
-        if (nch > 1) task_units = std::max(1, std::min(task_units, kTaskItems / nch));
         cp.task_units = task_units;
       }
       long long padded = 0;
@@ -1924,7 +1921,7 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
     double best = 0.0;
     if ((rc = time_class(i, &best))) return rc;
     if (eri_fast_has_plan(cp.la, cp.lb, cp.lc, cp.ld)) {
-      // candidate shapes: 4 or 8 wavefronts per workgroup, 8..64 wave units
+      // candidate shapes: 4 or 8 wavefronts per workgroup, 16..128 wave units
       // per row task, LDS row accumulators on (where they fit) or off
       int cC = 0, cD = 0;
       const bool fits = row_acc_fits(ctx, cp.la, cp.lb, cp.lc, cp.ld, &cC, &cD);
@@ -1932,7 +1929,7 @@ int asuka_jk_autotune(asuka_jk_ctx* ctx, const double* D, int flags, int reps, v
       bool ba = cp.acc_colsC > 0;
       for (int acc = fits ? 1 : 0; acc >= 0; --acc)
         for (int wpg : {4, 8})
-          for (int tu : {8, 16, 32, 64}) {
+          for (int tu : {16, 32, 64, 128}) {
             if ((rc = apply_shape(ctx, i, wpg, tu, acc != 0, s))) return rc;
             double ms = 0.0;
             if ((rc = time_class(i, &ms))) return rc;
