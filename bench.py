@@ -40,7 +40,7 @@ METRIC = "Fock-build s & quartets/s: mixed vs FP64 GPU at ≤1e-6 Eh; gradient s
 # by what the counter reads for a 1 GiB device copy), measured with rocprofv3
 # --pmc per workload and mode (profiles/*_hbm_traffic.md); None where not
 # measured.
-HBM_TRAFFIC = {(2, "fp64"): 1.269e10, (2, "mixed"): 1.625e10, (5, "fp64"): 5.505e11, (5, "mixed"): 5.176e11}
+HBM_TRAFFIC = {(2, "fp64"): 1.247e10, (2, "mixed"): 1.625e10, (5, "fp64"): 5.505e11, (5, "mixed"): 5.176e11}
 
 CONFIGS = {
     2: dict(n=8, seed=2, basis="cc-pVTZ", nocc=40, cseed=1002, eps=1e-12, kind="jk",
